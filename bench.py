@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused FP64 RK4 step on one MI355X.
+
+Workload: (phi, Pi) wave at 256^3, 8th-order stencils (n=4), Kreiss-Oliger
+order 4 with sigma=0.2, 16-cell quadratic absorbing layer (max rate 2.0) with
+Dirichlet-0 ghosts on a non-periodic unit cube, dt = dx/4; initial data are 8
+seeded Gaussians. A "step" is one RK4 step of the whole box (four fused stage
+kernels), with the state resident in HBM; steps are replayed from a captured
+hipGraph.
+
+Prints one JSON line:
+  value      cell-updates/s = 256^3 * steps / wall time between synchronisations
+  roofline   algorithmic HBM bytes per stage launch (128 * ncomp B per cell per
+             RK4 step, i.e. 32 * ncomp B per cell per launch on average) divided
+             by the mean stage-kernel duration from HIP events recorded on the
+             solver's stream around the timed steps, against 8 TB/s
+  cpu_baseline  the CPU oracle (a restatement of the reference arithmetic,
+             OpenMP over z planes) on the same 256^3 workload, a bounded number
+             of steps on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per stage launch measured with rocprofv3 counters, if a summary was committed."""
+    p = ROOT / "profiles" / "fused_stage_wave256_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(case, state, budget_s: float):
+    from oracle import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.rk4_steps(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or el / steps * (steps + 1) > 2.0 * budget_s:
+            break
+    o.close()
+    return {
+        "value": case.cells * steps / el,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} RK4 step(s) of the same 256^3 workload ({case.cells} cells) by the C oracle, "
+                  f"OpenMP {threads} threads, {el:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cells per side (default: the 256^3 workload)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle stepping")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build has no "
+              "multi-GPU path: each solve runs on a single MI355X)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from treesap_amd import Solver
+    from treesap_amd.workloads import wave256
+
+    case, state, _ = wave256(n=args.n)
+    cells = case.cells
+    log(f"[bench] workload (phi,Pi) {args.n}^3 n=4 KO4 sigma=0.2 ABL16, {cells} cells")
+
+    s = Solver(case)
+    s.set_state(state)
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        cfg = s.autotune(3)
+        log(f"[bench] autotune chose {cfg} in {time.perf_counter() - t0:.1f} s")
+    cfg = s.kernel
+
+    s.graph_rk4_steps(args.warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = s.time_rk4(args.steps, use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = s.get_state()
+    if not np.isfinite(out).all():
+        log("[bench] non-finite state after the timed steps")
+        return 1
+
+    # per-stage kernel durations (eager launches, events around each launch)
+    stage_ms, fill_ms = s.profile_rk4(10)
+    bytes_per_cell = s.rk4_bytes_per_cell()  # 128 * ncomp
+    s.close()
+
+    value = cells * args.steps / wall
+    mean_launch_s = dev_ms / 1e3 / (4 * args.steps)
+    bytes_per_launch = bytes_per_cell * cells / 4
+    achieved = bytes_per_launch / mean_launch_s / 1e9
+    stage_gbs = [
+        (b * case.ncomp * cells) / (ms / 1e3 / 10) / 1e9
+        for b, ms in zip((24, 40, 40, 24), stage_ms)
+    ]
+    log(f"[bench] {args.steps} steps: wall {wall * 1e3:.1f} ms, device {dev_ms:.1f} ms, "
+        f"{value:.3e} cell-updates/s, per-stage GB/s {[round(x) for x in stage_gbs]}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle baseline ...")
+        cpu = cpu_baseline(case, state, args.cpu_budget)
+        log(f"[bench] CPU: {cpu['value']:.3e} cell-updates/s ({cpu['sample']})")
+
+    line = {
+        "metric": "RK4 cell-updates/s & % HBM roofline (FP64); batched vs plain; adjoint/fwd time",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"wave256: (phi,Pi) wave {args.n}^3 FP64, 8th-order stencils (n=4), KO order 4 sigma=0.2, "
+                        "16-cell quadratic ABL (max 2.0), Dirichlet 0, dt=dx/4, 8 seeded Gaussians",
+            "cells": cells,
+            "kernel": cfg,
+            "graph": True,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "mean_stage_launch_us": mean_launch_s * 1e6,
+            "stage_gbs_eager": stage_gbs,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

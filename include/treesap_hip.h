@@ -1,0 +1,182 @@
+/*
+ * treesap_hip.h -- C ABI of the MI355X (gfx950) explicit time-stepping library.
+ *
+ * The reference (TreESaP) has no C/FFI boundary: its plug-in API is the
+ * compile-time C++ concept set of pde-model/PDEModelConcepts.hpp:44-139 and
+ * the integrator lifecycle of Integrator.hpp:75-106 / SingleLevelIntegrator.hpp.
+ * Each entry point below replaces one piece of that lifecycle; the comment on
+ * each names the reference interface it stands in for. The C++ drop-in headers
+ * under include/treesap/ (PDEModel, SingleLevelRK4Integrator, ...) call these
+ * entry points, as do the Python binding and the tests.
+ *
+ * Conventions
+ *   - Every function returns a tsp_status; on failure tsp_last_error() returns
+ *     a thread-local message. Nothing throws across this boundary.
+ *   - A solver owns its device memory and is bound to one HIP stream. Calls on
+ *     one solver must not overlap (no internal locking). Host buffers passed in
+ *     are borrowed for the duration of the call only.
+ *   - State arrays exchanged through tsp_set_state/tsp_get_state hold the valid
+ *     region only, FP64, component-slowest then z, y, x fastest:
+ *       a[((c * nz + k) * ny + j) * nx + i].
+ *   - All arithmetic is IEEE FP64 without FMA contraction, in the reference's
+ *     operation order (amrex::Real is double).
+ */
+#ifndef TREESAP_HIP_H
+#define TREESAP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSP_MAX_COMP 12
+#define TSP_VERSION "0.1.0"
+
+typedef enum tsp_status {
+    TSP_OK = 0,
+    TSP_INVALID_ARG = 1,
+    TSP_OOM = 2,
+    TSP_HIP_ERROR = 3,
+    TSP_UNSUPPORTED = 4
+} tsp_status;
+
+/* Registered right-hand sides (the RHS functor is a compile-time type in the
+ * reference, RHSConcept at pde-model/PDEModelConcepts.hpp:44-64; the library
+ * carries device kernels for these systems). */
+typedef enum tsp_system {
+    TSP_SYS_WAVE_PHIPI = 0, /* (phi, Pi): phi_t = Pi, Pi_t = sum_a D_aa phi           */
+    TSP_SYS_WAVE_PHIU = 1,  /* (phi, u_a): examples/01-wave-eqn/WaveEqnRHS.hpp:16-50    */
+    TSP_SYS_NONLINEAR = 2,  /* (phi, pi): examples/02-temporaries/NonLinearRHS.hpp:16-71 */
+    TSP_SYS_TENSOR_ADM = 3  /* (h_ij, K_ij) linearised ADM, 12 comps, SymRank2 aliases   */
+} tsp_system;
+
+/* Physical boundary functor on non-periodic faces (BCConcept,
+ * PDEModelConcepts.hpp:70-90): NoOpBC (bcs/NoOpBC.hpp:22-47) leaves ghost
+ * memory untouched; DirichletBC (bcs/DirichletBC.hpp:21-66) writes target[]. */
+typedef enum tsp_bc { TSP_BC_NOOP = 0, TSP_BC_DIRICHLET = 1 } tsp_bc;
+
+/* ABLProfileConcept (rhs-decorators/ABLDecorator.hpp:24-29) instances. */
+typedef enum tsp_profile { TSP_PROFILE_LINEAR = 0, TSP_PROFILE_QUADRATIC = 1, TSP_PROFILE_CUBIC = 2 } tsp_profile;
+
+/* Register roles (SingleLevelIntegrator.hpp:332-333 and
+ * integrators/SingleLevelRK4Integrator.hpp:162-166). */
+typedef enum tsp_role { TSP_ROLE_OLD = 0, TSP_ROLE_NEW = 1, TSP_ROLE_ARG = 2, TSP_ROLE_ARG_NEXT = 3 } tsp_role;
+
+/* One PDE model instance: the runtime content of PDEModel
+ * (pde-model/PDEModel.hpp:70-77), KODecorator (KODecorator.hpp:32-74),
+ * ABLPDEModel/ABLDecorator::ABLData (ABLPDEModel.hpp:47-50,
+ * ABLDecorator.hpp:58-77) and amrex::Geometry. */
+typedef struct tsp_desc {
+    int32_t system;         /* tsp_system */
+    int32_t n[3];           /* valid cells per axis (Geometry::Domain) */
+    int32_t stencil_n;      /* derivative half-width (treesap::stencil ngrow template) */
+    int32_t ghost;          /* ghost width G = RHS::ngrow, >= stencil_n and >= ko_order */
+    int32_t ko_order;       /* KODecorator order r; 0 = no KO decorator */
+    int32_t bc;             /* tsp_bc */
+    double ko_strength;     /* KODecorator::strength */
+    double prob_lo[3];      /* RealBox lo */
+    double prob_hi[3];      /* RealBox hi */
+    int32_t periodic[3];    /* Geometry is_periodic */
+    int32_t abl_thickness;  /* ABLData::thickness in cells; 0 = no ABL decorator */
+    int32_t abl_profile;    /* tsp_profile */
+    int32_t reserved0;
+    double abl_max;         /* ABLData::max_strength */
+    double target[TSP_MAX_COMP]; /* ABLData::target_values == DirichletBC::boundary_values */
+    double gamma;           /* NonLinearRHS::gamma (system 2 only) */
+    double dt;              /* PDEModel::dt_hint() */
+} tsp_desc;
+
+/* Stage-kernel configuration chosen by tsp_autotune or set by hand. Every
+ * configuration produces bit-identical results; only the speed differs. */
+typedef struct tsp_kernel_cfg {
+    int32_t ghost_mode; /* 0: virtual ghosts (no fill pass); 1: FillPatch pass then plain reads */
+    int32_t tile_y;     /* rows (waves) per workgroup: 2, 4 or 8; tile_x is 64 */
+    int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
+    int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
+} tsp_kernel_cfg;
+
+typedef struct tsp_solver tsp_solver;
+
+/* Thread-local message describing the last failure on this thread. */
+const char *tsp_last_error(void);
+const char *tsp_version(void);
+
+/* Fill a descriptor with defaults: periodic unit cube, no decorators. */
+tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n);
+
+/* Replaces SingleLevelRK4Integrator(model, block_size)
+ * (integrators/SingleLevelRK4Integrator.hpp:65-67, SingleLevelIntegrator.hpp:197-199):
+ * validates the model, allocates the four registers (state_old, state_new,
+ * arg, arg_next) in device memory. `stream` is a hipStream_t or NULL (the
+ * library then creates its own default-priority stream). Validation happens
+ * before any HIP call. */
+tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out);
+tsp_status tsp_destroy(tsp_solver *s);
+
+/* Replaces SingleLevelIntegrator::initialise_impl (SingleLevelIntegrator.hpp:75-93):
+ * the initial condition is evaluated by the caller and uploaded into state_old.
+ * on_device != 0 means `src` is a device pointer. Resets time and step count. */
+tsp_status tsp_set_state(tsp_solver *s, const double *src, int32_t on_device);
+/* The reference has no state accessor (plotfiles only, PlotExporter.hpp:54-59). */
+tsp_status tsp_get_state(tsp_solver *s, double *dst, int32_t on_device);
+
+/* Replaces SingleLevelIntegrator::FillPatch (SingleLevelIntegrator.hpp:153-163)
+ * on the register with role `role` (tsp_role): periodic images then the
+ * physical-boundary functor, faces, edges and corners. */
+tsp_status tsp_fill_ghosts(tsp_solver *s, int32_t role);
+/* Copy a whole ghosted register (or its valid region) to host, for tests. */
+tsp_status tsp_get_register(tsp_solver *s, int32_t role, double *dst_valid_plus_ghost);
+
+/* Replaces `nsteps` calls of Integrator::step() (Integrator.hpp:91-106) on
+ * SingleLevelRK4Integrator::computeNewState (SingleLevelRK4Integrator.hpp:95-157). */
+tsp_status tsp_rk4_steps(tsp_solver *s, int64_t nsteps);
+/* Same on SingleLevelEulerIntegrator::computeNewState (SingleLevelEulerIntegrator.hpp:66-88). */
+tsp_status tsp_euler_steps(tsp_solver *s, int64_t nsteps);
+
+/* Integrator::time / Integrator::steps (Integrator.hpp:147-148). */
+tsp_status tsp_get_time(tsp_solver *s, double *time, int64_t *steps);
+
+tsp_status tsp_get_kernel(tsp_solver *s, tsp_kernel_cfg *cfg);
+tsp_status tsp_set_kernel(tsp_solver *s, const tsp_kernel_cfg *cfg);
+
+/* Time candidate stage-kernel configurations on this solver's shape (the state
+ * is saved and restored) and keep the fastest; results are cached per
+ * (device arch, system, stencil width, KO order, shape, ghost mode) in the
+ * file named by TSP_TUNE_CACHE, if set. `steps` RK4 steps are timed per
+ * candidate. The chosen configuration is written to *chosen if non-NULL. */
+tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen);
+
+/* hipGraph replay of RK4 steps (no reference counterpart). The register
+ * rotation of the reference has a period of two steps, so one graph holds two
+ * steps; tsp_graph_rk4_steps replays it and runs an odd remainder eagerly.
+ * Results are bit-identical to tsp_rk4_steps. */
+tsp_status tsp_graph_rk4_steps(tsp_solver *s, int64_t nsteps);
+
+/* Run `nsteps` eager RK4 steps recording HIP events around every stage
+ * launch on the solver's stream; writes the summed device time of each stage
+ * kernel (ms, 4 entries) and of any ghost-fill launches (ms, 1 entry). */
+tsp_status tsp_profile_rk4(tsp_solver *s, int64_t nsteps, double *stage_ms, double *fill_ms);
+
+/* Run `nsteps` RK4 steps (graph replay if use_graph != 0, else eager) with one
+ * pair of HIP events recorded on the solver's stream around them; waits for
+ * the stop event and writes the elapsed device time in ms. */
+tsp_status tsp_time_rk4(tsp_solver *s, int64_t nsteps, int32_t use_graph, double *ms);
+
+/* Block until the solver's stream is idle. */
+tsp_status tsp_sync(tsp_solver *s);
+
+/* Device pointer of the register with the given role and its layout: element
+ * (i,j,k,c), ghosts included (-G <= i < n+G), is at
+ *   ptr[i + j*sj + k*sk + c*sc]   with ptr pointing at (0,0,0,0). */
+tsp_status tsp_register_view(tsp_solver *s, int32_t role, double **ptr, int64_t *sj, int64_t *sk, int64_t *sc);
+
+/* Algorithmic HBM bytes of one RK4 step per valid cell (128 * ncomp). */
+int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TREESAP_HIP_H */

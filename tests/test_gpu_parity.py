@@ -1,0 +1,211 @@
+"""GPU parity: the HIP library against the CPU oracle on identical seeded inputs.
+
+The bar is the reference's: <= 1e-12 relative L2 error per component after the
+stated step count. Both sides evaluate the same expressions in the same order
+without FMA contraction, so the tests also require bitwise equality where the
+arithmetic is identical, which pins the kernels much harder than the tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from treesap_amd import Solver, _lib
+from treesap_amd.workloads import small_case, wave64, wave256
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2_per_comp(got, want):
+    out = []
+    for c in range(got.shape[0]):
+        den = np.linalg.norm(want[c])
+        num = np.linalg.norm(got[c] - want[c])
+        out.append(num / den if den > 0 else num)
+    return np.array(out)
+
+
+def run_both(case, state, nsteps, scheme="rk4", kernel=None, graph=False):
+    s = Solver(case)
+    if kernel:
+        s.kernel = kernel
+    s.set_state(state)
+    if scheme == "rk4":
+        (s.graph_rk4_steps if graph else s.rk4_steps)(nsteps)
+    else:
+        s.euler_steps(nsteps)
+    got = s.get_state()
+    t, n = s.time, s.steps
+    s.close()
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    (o.rk4_steps if scheme == "rk4" else o.euler_steps)(nsteps)
+    want = o.get_state()
+    assert n == o.steps and t == o.time
+    return got, want
+
+
+CASES = [
+    # system, n, stencil, ko, periodic, abl
+    ("phipi", (24, 20, 16), 2, 0, (1, 1, 1), 0),
+    ("phipi", (24, 20, 16), 2, 3, (1, 1, 1), 0),
+    ("phipi", (70, 21, 19), 4, 4, (0, 0, 0), 5),
+    ("phipi", (33, 18, 22), 3, 4, (1, 0, 1), 0),
+    ("phipi", (17, 9, 12), 1, 2, (0, 1, 0), 3),
+    ("phiu", (24, 20, 16), 1, 0, (1, 1, 1), 0),
+    ("phiu", (30, 14, 18), 3, 3, (0, 0, 0), 4),
+    ("nonlinear", (24, 20, 16), 2, 2, (1, 1, 1), 0),
+    ("nonlinear", (66, 12, 14), 4, 0, (0, 0, 0), 3),
+]
+
+
+@pytest.mark.parametrize("system,n,sn,ko,per,abl", CASES)
+@pytest.mark.parametrize("ghost_mode", [0, 1])
+def test_rk4_matches_oracle(system, n, sn, ko, per, abl, ghost_mode):
+    _say(f"[parity] rk4 {system} n={n} stencil={sn} ko={ko} periodic={per} abl={abl} ghost_mode={ghost_mode}")
+    case, state = small_case(system, n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl,
+                             gamma=0.3 if system == "nonlinear" else 0.0)
+    got, want = run_both(case, state, 7, kernel={"ghost_mode": ghost_mode})
+    err = rel_l2_per_comp(got, want)
+    assert np.all(err <= REL_TOL), err
+    assert np.array_equal(got, want), f"max abs diff {np.max(np.abs(got - want))}"
+
+
+@pytest.mark.parametrize("system,n,sn,ko,per,abl", CASES[:4] + CASES[5:7])
+def test_euler_matches_oracle(system, n, sn, ko, per, abl):
+    _say(f"[parity] euler {system} n={n}")
+    case, state = small_case(system, n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl)
+    case = case.replace(dt=0.05 * case.dt)
+    got, want = run_both(case, state, 5, scheme="euler")
+    assert np.all(rel_l2_per_comp(got, want) <= REL_TOL)
+    assert np.array_equal(got, want)
+
+
+TILE_CFGS = [
+    {"ghost_mode": 0, "tile_y": 2, "zchunk": 3, "xcd_remap": 0},
+    {"ghost_mode": 0, "tile_y": 8, "zchunk": 64, "xcd_remap": 1},
+    {"ghost_mode": 1, "tile_y": 4, "zchunk": 5, "xcd_remap": 1},
+    {"ghost_mode": 1, "tile_y": 2, "zchunk": 1, "xcd_remap": 0},
+]
+
+
+def test_every_tile_shape_is_bitwise_identical():
+    _say("[parity] tile shapes")
+    case, state = small_case("phipi", n=(130, 27, 23), stencil_n=4, ko_order=4, periodic=(0, 1, 0), abl=4)
+    outs = [run_both(case, state, 4, kernel=cfg)[0] for cfg in TILE_CFGS]
+    _, want = run_both(case, state, 4)
+    for o in outs:
+        assert np.array_equal(o, want)
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 5, 6])
+def test_graph_replay_equals_eager(nsteps):
+    _say(f"[parity] graph replay nsteps={nsteps}")
+    case, state = small_case("phipi", n=(40, 24, 20), stencil_n=2, ko_order=3, periodic=(1, 1, 1))
+    g, want = run_both(case, state, nsteps, graph=True)
+    assert np.array_equal(g, want)
+    # graph replays after an odd eager step (other parity) and after re-initialisation
+    s = Solver(case)
+    s.set_state(state)
+    s.rk4_steps(1)
+    s.graph_rk4_steps(nsteps)
+    s.set_state(state)
+    s.graph_rk4_steps(nsteps + 1)
+    got = s.get_state()
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    o.rk4_steps(nsteps + 1)
+    assert np.array_equal(got, o.get_state())
+
+
+@pytest.mark.parametrize("per,bc", [((1, 1, 1), "noop"), ((0, 0, 0), "dirichlet"), ((1, 0, 1), "dirichlet")])
+def test_ghost_fill_matches_oracle(per, bc):
+    _say(f"[parity] ghost fill periodic={per} bc={bc}")
+    case, state = small_case("phiu", n=(21, 13, 11), stencil_n=3, ko_order=4, periodic=per)
+    case = case.replace(bc=bc, target=tuple(0.5 + c for c in range(12)))
+    s = Solver(case)
+    s.set_state(state)
+    s.fill_ghosts(_lib.ROLE_OLD)
+    s.sync()
+    got = s.get_register(_lib.ROLE_OLD)
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    o.fill_ghosts(0)
+    assert np.array_equal(got, o.get_register(0))
+
+
+def test_wave64_full_workload_matches_oracle():
+    """64^3 periodic (phi, Pi), n=2, KO r=3 sigma=0.1, 100 RK4 steps."""
+    _say("[parity] wave64 100 steps")
+    case, state, nsteps = wave64()
+    got, want = run_both(case, state, nsteps, graph=True)
+    err = rel_l2_per_comp(got, want)
+    _say(f"  rel L2 per comp {err}")
+    assert np.all(err <= REL_TOL)
+    assert np.array_equal(got, want)
+
+
+def test_wave256_full_size_one_step_matches_oracle():
+    """256^3, n=4, KO r=4, 16-cell ABL, Dirichlet: one RK4 step at full size against the oracle."""
+    _say("[parity] wave256 full size, 1 step")
+    case, state, _ = wave256()
+    got, want = run_both(case, state, 1)
+    assert np.all(rel_l2_per_comp(got, want) <= REL_TOL)
+    assert np.array_equal(got, want)
+
+
+def test_wave256_full_size_properties():
+    """Size-independent properties at 256^3 over 40 steps: linearity under exact scaling
+    by 2 and mirror symmetry of reflected data (all operators are reflection symmetric)."""
+    _say("[parity] wave256 properties")
+    case, state, _ = wave256()
+    sym = state + state[:, :, :, ::-1]  # mirror-symmetric in x
+    a = Solver(case)
+    a.set_state(sym)
+    a.graph_rk4_steps(40)
+    ra = a.get_state()
+    a.set_state(2.0 * sym)
+    a.graph_rk4_steps(40)
+    rb = a.get_state()
+    a.close()
+    assert np.array_equal(rb, 2.0 * ra)
+    assert np.array_equal(ra, ra[:, :, :, ::-1])
+    assert np.isfinite(ra).all() and np.abs(ra).max() > 1e-3
+
+
+def test_device_pointers_in_and_out():
+    """set/get state through device memory (torch tensors) equals the host path."""
+    torch = pytest.importorskip("torch")
+    case, state = small_case("phipi", n=(32, 16, 8), stencil_n=2, ko_order=2)
+    s = Solver(case)
+    s.set_state(torch.from_numpy(state).cuda())
+    s.rk4_steps(3)
+    out = torch.empty(case.shape, dtype=torch.float64, device="cuda")
+    s.get_state(out)
+    host = s.get_state()
+    assert np.array_equal(out.cpu().numpy(), host)
+
+
+def test_autotune_restores_state():
+    case, state = small_case("phipi", n=(64, 32, 32), stencil_n=4, ko_order=4, periodic=(0, 0, 0), abl=4)
+    s = Solver(case)
+    s.set_state(state)
+    s.rk4_steps(2)
+    before = s.get_state()
+    cfg = s.autotune(1)
+    assert cfg["tile_y"] in (2, 4, 8)
+    assert s.steps == 2
+    assert np.array_equal(s.get_state(), before)
+    s.rk4_steps(3)
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    o.rk4_steps(5)
+    assert np.array_equal(s.get_state(), o.get_state())

@@ -1,0 +1,294 @@
+"""Pins the CPU oracle (oracle/oracle.c) against known answers, closed forms and
+brute force. No GPU. Each check is chosen so that a plausible slip in the
+oracle (a dropped term, a wrong sign or index, a transposed operand, a wrong
+grid factor) fails at least one of them."""
+import itertools
+import json
+import math
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import fourier
+from oracle import oracle as orc
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "central_differences.json"
+
+
+def _case(**kw):
+    base = dict(system="phipi", n=(12, 10, 8), stencil_n=2, ghost=2, ko_order=0, ko_strength=0.0,
+                prob_lo=(0.0, 0.0, 0.0), prob_hi=(1.0, 1.0, 1.0), periodic=(1, 1, 1), bc="noop",
+                abl_thickness=0, abl_max=0.0, abl_profile="quadratic", target=(0.0,) * 12, gamma=0.0, dt=0.01)
+    base.update(kw)
+    return base
+
+
+# ---------------------------------------------------------------------------
+# weights
+# ---------------------------------------------------------------------------
+def test_weights_match_textbook_table():
+    g = json.loads(GOLDEN.read_text())
+    for kind, idx in (("first", 0), ("second", 1), ("third", 2)):
+        for n, vals in g[kind].items():
+            n = int(n)
+            w = orc.weights(n)[idx]
+            want = [Fraction(v) for v in vals]
+            if kind == "second":
+                got = w[0:n + 1]
+            else:
+                got = w[1:n + 1]
+            assert len(got) == len(want)
+            for a, b in zip(got, want):
+                assert a == pytest.approx(float(b), rel=2e-15, abs=0), (kind, n, got, want)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_weights_match_fornberg(n):
+    w1, w2, w3 = orc.weights(n)
+    f1 = fourier.central_weights(1, n)
+    f2 = fourier.central_weights(2, n)
+    for p in range(1, n + 1):
+        # single rounding of an exact rational: correctly rounded
+        assert w1[p] == float(f1[p])
+        assert w2[p] == float(f2[p])
+        assert orc.lib().or_w1(n, -p) == -w1[p]
+    assert w2[0] == pytest.approx(float(f2[0]), rel=4e-16)
+    if n >= 2:
+        f3 = fourier.central_weights(3, n)
+        for p in range(1, n + 1):
+            assert w3[p] == pytest.approx(float(f3[p]), rel=1e-14)
+
+
+def test_harmonic_and_binomial():
+    L = orc.lib()
+    for n in range(1, 8):
+        assert L.or_harmonic(n, 2) == pytest.approx(sum(1.0 / k ** 2 for k in range(1, n + 1)), rel=1e-16)
+        for k in range(0, n + 1):
+            assert L.or_binom(n, k) == math.comb(n, k)
+
+
+@pytest.mark.parametrize("r", [1, 2, 3, 4])
+def test_ko_coefficients_and_symbol(r):
+    g = json.loads(GOLDEN.read_text())["kreiss_oliger"][str(r)]
+    coefs = orc.ko_coefs(r)
+    assert coefs[0] == g["centre"]
+    assert coefs[1:] == g["offsets"]
+    exact = fourier.ko_weights(r)
+    for s in range(r + 1):
+        assert Fraction(coefs[s], 4 ** r) == exact[s]
+    # symbol: norm * (c0 + 2 sum c_s cos(s t)) = -sin^{2r}(t/2)
+    for t in np.linspace(0, math.pi, 17):
+        sym = (coefs[0] + 2 * sum(coefs[s] * math.cos(s * t) for s in range(1, r + 1))) / 4 ** r
+        assert sym == pytest.approx(-math.sin(t / 2) ** (2 * r), abs=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# polynomial exactness of the point stencils (interior, no ghosts involved)
+# ---------------------------------------------------------------------------
+def _poly_case(n_st, dx=(0.1, 0.125, 0.2)):
+    N = 4 * n_st + 3
+    hi = tuple(N * d for d in dx)
+    return _case(n=(N, N, N), stencil_n=n_st, ghost=n_st, prob_hi=hi, periodic=(0, 0, 0), bc="dirichlet")
+
+
+def _centres(case):
+    n = case["n"]
+    return [(np.arange(n[a]) + 0.5) * (case["prob_hi"][a] / n[a]) for a in range(3)]
+
+
+@pytest.mark.parametrize("n_st", [1, 2, 3, 4])
+def test_first_and_second_derivative_polynomial_exactness(n_st):
+    case = _poly_case(n_st)
+    xs = _centres(case)
+    N = case["n"][0]
+    rng = np.random.default_rng(n_st)
+    deg = 2 * n_st
+    coef = rng.uniform(-1, 1, size=(deg + 2,))
+    s = orc.OracleSolver(case)
+    for axis in range(3):
+        # p(x_axis) of degree 2n (D1 exact) and 2n+1 (D2 exact)
+        for d, kind in ((deg, 1), (deg + 1, 2)):
+            c = coef[: d + 1]
+            field = np.zeros(s.shape)
+            grid = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")[2 - axis]
+            field[0] = np.polynomial.polynomial.polyval(grid, c)
+            s.set_state(field)
+            s.fill_ghosts(0)
+            i = j = k = N // 2
+            x0 = xs[axis][N // 2]
+            dc = np.polynomial.polynomial.polyder(c, kind)
+            want = np.polynomial.polynomial.polyval(x0, dc)
+            got = s.point_derivative([axis] * kind, i, j, k)
+            scale = max(1.0, abs(want), np.abs(np.polynomial.polynomial.polyval(x0, c)))
+            assert got == pytest.approx(want, abs=1e-9 * scale), (axis, kind)
+            # one degree more is NOT exact for D1 (truncation term present)
+            if kind == 1:
+                c2 = np.zeros(deg + 2)
+                c2[deg + 1] = 1.0
+                field[0] = np.polynomial.polynomial.polyval(grid, c2)
+                s.set_state(field)
+                got2 = s.point_derivative([axis], i, j, k)
+                want2 = (deg + 1) * x0 ** deg
+                assert abs(got2 - want2) > 1e-9 * max(1.0, abs(want2))
+
+
+@pytest.mark.parametrize("n_st", [2, 3])
+def test_mixed_and_third_derivatives_polynomial(n_st):
+    case = _poly_case(n_st)
+    xs = _centres(case)
+    N = case["n"][0]
+    s = orc.OracleSolver(case)
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    field = np.zeros(s.shape)
+    # f = x^2 y^3 z + 3 x y - z^2 x^3
+    field[0] = X ** 2 * Y ** 3 * Z + 3 * X * Y - Z ** 2 * X ** 3
+    s.set_state(field)
+    c = N // 2
+    x, y, z = xs[0][c], xs[1][c], xs[2][c]
+    want = {
+        (0, 1): 2 * x * 3 * y ** 2 * z + 3,
+        (1, 0): 2 * x * 3 * y ** 2 * z + 3,
+        (0, 2): 2 * x * y ** 3 - 2 * z * 3 * x ** 2,
+        (2, 0): 2 * x * y ** 3 - 2 * z * 3 * x ** 2,
+        (1, 2): x ** 2 * 3 * y ** 2,
+        (2, 1): x ** 2 * 3 * y ** 2,
+        (0, 0, 1): 2 * 3 * y ** 2 * z,
+        (0, 1, 2): 2 * x * 3 * y ** 2,
+        (0, 0, 0): -6 * z ** 2,
+        (1, 1, 1): 6 * x ** 2 * z,
+    }
+    for dirs, w in want.items():
+        got = s.point_derivative(list(dirs), c, c, c)
+        assert got == pytest.approx(w, rel=1e-8, abs=1e-8), dirs
+
+
+# ---------------------------------------------------------------------------
+# ABL sigma and ghost fill by brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("profile", ["linear", "quadratic", "cubic"])
+def test_abl_sigma_brute_force(profile):
+    n = (11, 9, 10)
+    T = 3
+    case = _case(n=n, periodic=(0, 0, 0), bc="dirichlet", abl_thickness=T, abl_max=1.7, abl_profile=profile)
+    s = orc.OracleSolver(case)
+    bulk = [(a, b, c) for a in range(T, n[0] - T) for b in range(T, n[1] - T) for c in range(T, n[2] - T)]
+    bulk = np.array(bulk)
+    p = {"linear": lambda v: v, "quadratic": lambda v: v * v, "cubic": lambda v: v ** 3}[profile]
+    for i, j, k in itertools.product(range(n[0]), range(n[1]), range(n[2])):
+        d = np.sqrt(((bulk - np.array([i, j, k])) ** 2).sum(axis=1)).min()
+        want = 0.0 if d == 0 else 1.7 * p(min(d / T, 1.0))
+        assert s.abl_sigma(i, j, k) == pytest.approx(want, rel=1e-15, abs=0)
+
+
+@pytest.mark.parametrize("periodic,bc", [((1, 1, 1), "noop"), ((0, 0, 0), "dirichlet"), ((1, 0, 1), "dirichlet"),
+                                         ((0, 1, 0), "noop")])
+def test_ghost_fill_brute_force(periodic, bc):
+    n = (7, 6, 5)
+    G = 3
+    case = _case(system="phiu", n=n, stencil_n=2, ghost=G, periodic=periodic, bc=bc,
+                 target=tuple(0.25 * (c + 1) for c in range(12)))
+    s = orc.OracleSolver(case)
+    rng = np.random.default_rng(7)
+    st = rng.standard_normal(s.shape)
+    s.set_state(st)
+    before = s.get_register(0)
+    s.fill_ghosts(0)
+    reg = s.get_register(0)
+    # expected: ghosts outside along a non-periodic axis -> target (Dirichlet) or untouched (NoOp)
+    # otherwise the periodic image of a valid cell
+    for c in range(4):
+        for kk, jj, ii in itertools.product(range(n[2] + 2 * G), range(n[1] + 2 * G), range(n[0] + 2 * G)):
+            iv = (ii - G, jj - G, kk - G)
+            inside = all(0 <= iv[a] < n[a] for a in range(3))
+            if inside:
+                assert reg[c, kk, jj, ii] == st[c, iv[2], iv[1], iv[0]]
+                continue
+            phys = any(not periodic[a] and not (0 <= iv[a] < n[a]) for a in range(3))
+            if phys:
+                want = 0.25 * (c + 1) if bc == "dirichlet" else before[c, kk, jj, ii]
+            else:
+                w = [iv[a] % n[a] for a in range(3)]
+                want = st[c, w[2], w[1], w[0]]
+            assert reg[c, kk, jj, ii] == want
+
+
+# ---------------------------------------------------------------------------
+# end to end: single Fourier mode closed form on periodic grids
+# ---------------------------------------------------------------------------
+SYSTEM_CASES = [
+    ("phipi", 2, 0), ("phipi", 2, 3), ("phipi", 4, 4), ("phipi", 1, 1),
+    ("phiu", 1, 0), ("phiu", 3, 3),
+    ("nonlinear", 2, 2),
+    ("tensor", 2, 0), ("tensor", 3, 3),
+]
+
+
+@pytest.mark.parametrize("system,n_st,ko", SYSTEM_CASES)
+@pytest.mark.parametrize("scheme", ["rk4", "euler"])
+def test_fourier_mode_closed_form(system, n_st, ko, scheme):
+    n = (16, 12, 10)
+    G = max(n_st, ko)
+    case = _case(system=system, n=n, stencil_n=n_st, ghost=G, ko_order=ko, ko_strength=0.17 if ko else 0.0,
+                 prob_hi=(1.0, 0.8, 1.25), dt=0.2 / 16)
+    s = orc.OracleSolver(case)
+    rng = np.random.default_rng(hash((system, n_st, ko)) % 2 ** 32)
+    for mode in [(1, 0, 0), (2, -1, 1), (3, 2, -2)]:
+        amps = rng.standard_normal(s.ncomp) + 1j * rng.standard_normal(s.ncomp)
+        init = fourier.mode_field(case, mode, amps)
+        s.set_state(init)
+        nsteps = 6
+        (s.rk4_steps if scheme == "rk4" else s.euler_steps)(nsteps)
+        got = s.get_state()
+        want = fourier.evolve(case, mode, amps, nsteps, scheme)
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err < 1e-12, (mode, err)
+
+
+def test_fourier_closed_form_is_sensitive():
+    """A KO strength off by 1% is visible to the closed-form check (the check has teeth)."""
+    case = _case(n=(16, 12, 10), stencil_n=2, ghost=3, ko_order=3, ko_strength=0.2, dt=0.2 / 16)
+    s = orc.OracleSolver(case)
+    amps = np.array([1.0 + 0.5j, -0.3j])
+    s.set_state(fourier.mode_field(case, (3, 2, 1), amps))
+    s.rk4_steps(6)
+    wrong = dict(case, ko_strength=0.202)
+    want_wrong = fourier.evolve(wrong, (3, 2, 1), amps, 6)
+    err = np.linalg.norm(s.get_state() - want_wrong) / np.linalg.norm(want_wrong)
+    assert err > 1e-6
+
+
+def test_time_and_step_counter():
+    case = _case(dt=0.1)
+    s = orc.OracleSolver(case)
+    s.set_state(np.zeros(s.shape))
+    s.rk4_steps(3)
+    t = 0.0
+    for _ in range(3):
+        t += 0.1
+    assert s.steps == 3 and s.time == t
+
+
+def test_nonlinear_term_matches_direct_evaluation():
+    """NonLinearRHS: pi_t = lap(phi) - gamma * pi * |grad phi|^2 (examples/02-temporaries/NonLinearRHS.hpp:39-71);
+    the gamma term is checked against numpy derivatives of a smooth periodic field."""
+    n = (24, 24, 24)
+    case = _case(system="nonlinear", n=n, stencil_n=4, ghost=4, gamma=0.7)
+    s = orc.OracleSolver(case)
+    lin = orc.OracleSolver(dict(case, system="phipi"))
+    X = (np.arange(24) + 0.5) / 24
+    Z, Y, Xg = np.meshgrid(X, X, X, indexing="ij")
+    st = np.zeros(s.shape)
+    st[0] = np.sin(2 * np.pi * Xg) * np.cos(2 * np.pi * Y) + 0.3 * np.sin(2 * np.pi * Z)
+    st[1] = 0.5 + 0.1 * np.cos(2 * np.pi * (Xg + Y))
+    s.set_state(st)
+    lin.set_state(st)
+    r = s.eval_rhs(0)
+    rl = lin.eval_rhs(0)
+    assert np.array_equal(r[0], st[1])
+    gx = 2 * np.pi * np.cos(2 * np.pi * Xg) * np.cos(2 * np.pi * Y)
+    gy = -2 * np.pi * np.sin(2 * np.pi * Xg) * np.sin(2 * np.pi * Y)
+    gz = 0.3 * 2 * np.pi * np.cos(2 * np.pi * Z)
+    want = rl[1] - 0.7 * st[1] * (gx ** 2 + gy ** 2 + gz ** 2)
+    assert np.max(np.abs(r[1] - want)) < 1e-3 * np.max(np.abs(want))

@@ -1,0 +1,154 @@
+// common.hpp -- definitions shared by the HIP kernels and the C-ABI host code.
+//
+// Stencil weights are the closed forms of util/stencil_impl.hpp:56-140 and
+// util/cexprmath.hpp:32-105, evaluated at compile time (IEEE double, round to
+// nearest) so every kernel carries them as immediates. The integer factors of
+// the Kreiss-Oliger operator follow util/stencil.hpp:96-103.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "treesap_hip.h"
+
+namespace tsp {
+
+// ---------------------------------------------------------------------------
+// compile-time unrolling (util/static_for.hpp:38-68 fixes the component order)
+// ---------------------------------------------------------------------------
+template <int Start, int End, class F> __host__ __device__ __forceinline__ constexpr void static_for(F &&f) {
+    if constexpr (Start < End) {
+        f.template operator()<Start>();
+        static_for<Start + 1, End>(f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// integer/double helpers (util/cexprmath.hpp)
+// ---------------------------------------------------------------------------
+namespace cx {
+// cexprmath.hpp:32-37
+constexpr int64_t factorial(int n) {
+    int64_t res = 1;
+    for (int i = 2; i <= n; ++i) res *= i;
+    return res;
+}
+// cexprmath.hpp:46-59
+constexpr int64_t binom(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    if (k == 0 || k == n) return 1;
+    if (k > n / 2) k = n - k;
+    int64_t res = 1;
+    for (int i = 1; i <= k; ++i) res = (res * (n - i + 1)) / i;
+    return res;
+}
+// cexprmath.hpp:68-77
+constexpr int64_t power(int64_t n, int k) {
+    int64_t result = 1;
+    while (k > 0) {
+        if (k & 1) result *= n;
+        n *= n;
+        k >>= 1;
+    }
+    return result;
+}
+// cexprmath.hpp:85
+constexpr int alternating_sign(int k) { return (k & 1) ? -1 : 1; }
+// cexprmath.hpp:99-105
+constexpr double harmonic(int n, int m) {
+    double sum = 0.0;
+    for (int k = 1; k <= n; k++) sum += 1.0 / static_cast<double>(power(k, m));
+    return sum;
+}
+// stencil_impl.hpp:56-71
+constexpr double w1(int n, int p) {
+    const int ka = p > 0 ? p : -p;
+    if (p == 0) return 0;
+    const int64_t num = alternating_sign(ka + 1) * power(factorial(n), 2);
+    const int64_t den = ka * factorial(n - ka) * factorial(n + ka);
+    const double w = static_cast<double>(num) / static_cast<double>(den);
+    return p > 0 ? w : -w;
+}
+// stencil_impl.hpp:91-104
+constexpr double w2(int n, int p) {
+    const int k = p > 0 ? p : -p;
+    if (p == 0) return -2.0 * harmonic(n, 2);
+    const int64_t num = 2 * alternating_sign(k + 1) * power(factorial(n), 2);
+    const int64_t den = k * k * factorial(n - k) * factorial(n + k);
+    return static_cast<double>(num) / static_cast<double>(den);
+}
+// stencil_impl.hpp:124-140
+constexpr double w3(int n, int p) {
+    const int ka = p > 0 ? p : -p;
+    if (p == 0) return 0;
+    const int64_t num = 6 * alternating_sign(ka + 1) * power(factorial(n), 2);
+    const int64_t den = power(ka, 3) * factorial(n - ka) * factorial(n + ka);
+    const double w = ((1.0 - static_cast<double>(ka * ka) * harmonic(n, 2)) * static_cast<double>(num) /
+                      static_cast<double>(den));
+    return p > 0 ? w : -w;
+}
+// stencil.hpp:96-101: centre -binom(2r,r); offset s: -alternating_sign(s)*binom(2r,r+s)
+constexpr int64_t ko_coef(int r, int s) {
+    if (s == 0) return -binom(2 * r, r);
+    return -alternating_sign(s) * binom(2 * r, r + s);
+}
+// stencil.hpp:93
+constexpr double ko_norm(int r) { return 1.0 / static_cast<double>(power(2, 2 * r)); }
+} // namespace cx
+
+// ---------------------------------------------------------------------------
+// stage arguments (one RK4 stage or one Euler update over one box)
+// ---------------------------------------------------------------------------
+enum StageKind : int32_t { kStage0 = 0, kStage1 = 1, kStage2 = 2, kStage3 = 3, kEuler = 4 };
+
+struct StageArgs {
+    const double *src;       // register read with stencils (init for stage 0, arg otherwise)
+    const double *init;      // state_old (centre reads)
+    double *acc;             // state_new
+    double *argn;            // arg_next
+    const double *sigma_tab; // ABL sigma by squared distance to the bulk box
+    int32_t nx, ny, nz;
+    int32_t sj, sk, sc;      // element strides of the register layout
+    int32_t stage;           // StageKind
+    int32_t ghost_mode;      // 0 virtual ghosts, 1 read ghost memory
+    int32_t periodic[3];
+    int32_t dirichlet;
+    int32_t abl_T;           // ABL thickness (0: no ABL decorator)
+    int32_t zchunk;
+    int32_t tiles_x, tiles_y, nzc;
+    int32_t xcd_remap;
+    double b, a_next;        // dt*b_s and dt*a_{s+1} (integrators/SingleLevelRK4Integrator.hpp:119-121)
+    double gf1[3];           // 1/dx   (stencil_impl.hpp:213, stencil.hpp:94)
+    double gf2[3];           // 1/(dx*dx) (stencil_impl.hpp:255)
+    double ko_strength;
+    double gamma;
+    double target[TSP_MAX_COMP];
+};
+
+// Launch a fused stage kernel of one (system, stencil width, KO order) with
+// tile height `tile_y`; fills the tile counts of a copy of `a`.
+using LaunchFn = hipError_t (*)(const StageArgs &a, int tile_y, hipStream_t stream);
+
+LaunchFn lookup_phipi(int n, int r);
+LaunchFn lookup_phiu(int n, int r);
+LaunchFn lookup_nonlinear(int n, int r);
+
+// Ghost fill (SingleLevelIntegrator.hpp:153-163) of one register.
+struct GhostArgs {
+    double *p; // element (0,0,0,0)
+    int32_t nx, ny, nz, G, nc;
+    int32_t sj, sk, sc;
+    int32_t periodic[3];
+    int32_t dirichlet;
+    double target[TSP_MAX_COMP];
+};
+hipError_t launch_ghost_fill(const GhostArgs &g, hipStream_t stream);
+
+// valid-region pack/unpack between a dense [c][k][j][i] buffer and a register
+hipError_t launch_unpack(double *reg0, const double *dense, int nx, int ny, int nz, int nc, int sj, int sk, int sc,
+                         hipStream_t stream);
+hipError_t launch_pack(double *dense, const double *reg0, int nx, int ny, int nz, int nc, int sj, int sk, int sc,
+                       hipStream_t stream);
+
+} // namespace tsp

@@ -1,0 +1,396 @@
+// stage_kernel.hpp -- fused RK4 stage / Euler kernel for systems whose RHS
+// reads neighbours along the coordinate axes only (first and unmixed second
+// derivatives plus Kreiss-Oliger dissipation).
+//
+// What one launch computes is integrators/SingleLevelRK4Integrator.hpp:137-156
+// (or SingleLevelEulerIntegrator.hpp:245-251) over the whole box: per valid
+// cell, components in order 0..ncomp-1, k = F<c>(src) with F the decorated RHS
+// ABLDecorator<KODecorator<User>> (ABLDecorator.hpp:102-108,
+// KODecorator.hpp:58-70), then the stage axpys.
+//
+// Layout on the GPU (gfx950, wave64):
+//   * a workgroup owns a 64 x TY column tile in (x, y) and marches a chunk of
+//     z planes; wave w of the group owns x-row y0+w, lane l owns x0+l, so every
+//     global load and store of a wave is one contiguous 512-byte row;
+//   * z neighbours live in a per-lane register queue of 2H+1 planes (each
+//     plane of the source register is loaded from HBM once per chunk, one
+//     plane ahead of use);
+//   * x and y neighbours come from an LDS copy of the current plane with an
+//     H-wide halo along x and y (corners are not needed for axis stencils);
+//     the halo of the next plane is prefetched into registers while the
+//     current plane is computed;
+//   * ghosts are either virtual (out-of-domain indices are mapped: periodic
+//     wrap, Dirichlet constant) or read from ghost memory after a fill pass.
+// The arithmetic is the reference's expression for expression (weights times
+// differences, grid factor last, left-fold sums), so every tile shape gives
+// bit-identical results.
+#pragma once
+
+#include "common.hpp"
+
+namespace tsp {
+
+// ---------------------------------------------------------------------------
+// one-dimensional stencils on a cell view (util/stencil_impl.hpp:206-268,
+// util/stencil.hpp:77-104); V::at<C, AX, S>() is component C at offset S
+// along axis AX from the centre cell.
+// ---------------------------------------------------------------------------
+template <int N, int AX, int C, class V> __device__ __forceinline__ double D1(const V &v) {
+    double st = 0.0;
+    static_for<1, N + 1>([&]<int S>() {
+        constexpr double w = cx::w1(N, S);
+        st += w * (v.template at<C, AX, S>() - v.template at<C, AX, -S>());
+    });
+    return v.gf1(AX) * st;
+}
+
+template <int N, int AX, int C, class V> __device__ __forceinline__ double D2(const V &v) {
+    constexpr double w0 = cx::w2(N, 0);
+    double st = w0 * v.template at<C, AX, 0>();
+    static_for<1, N + 1>([&]<int S>() {
+        constexpr double w = cx::w2(N, S);
+        st += w * (v.template at<C, AX, S>() + v.template at<C, AX, -S>());
+    });
+    return v.gf2(AX) * st;
+}
+
+template <int R, int AX, int C, class V> __device__ __forceinline__ double KO(const V &v) {
+    constexpr double c0 = static_cast<double>(cx::ko_coef(R, 0));
+    constexpr double norm = cx::ko_norm(R);
+    double result = c0 * v.template at<C, AX, 0>();
+    static_for<1, R + 1>([&]<int S>() {
+        constexpr double cs = static_cast<double>(cx::ko_coef(R, S));
+        result += cs * (v.template at<C, AX, S>() + v.template at<C, AX, -S>());
+    });
+    return result * norm * v.gf1(AX);
+}
+
+// ---------------------------------------------------------------------------
+// registered axis-stencil systems
+// ---------------------------------------------------------------------------
+
+// (phi, Pi) wave: phi_t = Pi, Pi_t = contract_a D_aa phi (tensor_operations.hpp:45-61)
+struct SysPhiPi {
+    static constexpr int NC = 2;
+    static constexpr bool user_stencil(int c) { return c == 0; }
+    template <int N, int C, class V> __device__ __forceinline__ static double user(const V &v, const StageArgs &) {
+        if constexpr (C == 0) {
+            return v.template at<1, 0, 0>();
+        } else {
+            double result = 0.0;
+            result += D2<N, 0, 0>(v);
+            result += D2<N, 1, 0>(v);
+            result += D2<N, 2, 0>(v);
+            return result;
+        }
+    }
+};
+
+// examples/01-wave-eqn/WaveEqnRHS.hpp:37-45: phi_t = contract_a D_a u_a, (u_a)_t = D_a phi
+struct SysPhiU {
+    static constexpr int NC = 4;
+    static constexpr bool user_stencil(int) { return true; }
+    template <int N, int C, class V> __device__ __forceinline__ static double user(const V &v, const StageArgs &) {
+        if constexpr (C == 0) {
+            double result = 0.0;
+            result += D1<N, 0, 1>(v);
+            result += D1<N, 1, 2>(v);
+            result += D1<N, 2, 3>(v);
+            return result;
+        } else {
+            return D1<N, C - 1, 0>(v);
+        }
+    }
+};
+
+// examples/02-temporaries/NonLinearRHS.hpp:39-71
+struct SysNonLinear {
+    static constexpr int NC = 2;
+    static constexpr bool user_stencil(int c) { return c == 0; }
+    template <int N, int C, class V> __device__ __forceinline__ static double user(const V &v, const StageArgs &a) {
+        if constexpr (C == 0) {
+            return v.template at<1, 0, 0>(); // workspace.pi = pi.load(idx)
+        } else {
+            const double g0 = D1<N, 0, 0>(v);
+            const double g1 = D1<N, 1, 0>(v);
+            const double g2 = D1<N, 2, 0>(v);
+            double grad_sq = 0.0;
+            grad_sq += g0 * g0;
+            grad_sq += g1 * g1;
+            grad_sq += g2 * g2;
+            double laplacian = 0.0;
+            laplacian += D2<N, 0, 0>(v);
+            laplacian += D2<N, 1, 0>(v);
+            laplacian += D2<N, 2, 0>(v);
+            return (laplacian - a.gamma * v.template at<1, 0, 0>() * grad_sq);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// kernel
+// ---------------------------------------------------------------------------
+template <class Sys, int R> struct Staging {
+    static constexpr bool staged(int c) { return R > 0 || Sys::user_stencil(c); }
+    static constexpr int count() {
+        int n = 0;
+        for (int c = 0; c < Sys::NC; ++c) n += staged(c) ? 1 : 0;
+        return n;
+    }
+    static constexpr int slot(int c) {
+        int n = 0;
+        for (int i = 0; i < c; ++i) n += staged(i) ? 1 : 0;
+        return n;
+    }
+};
+
+// Load element (i,j,k,c) of a register, mapping ghost indices. Indices more
+// than H outside the domain are never consumed by a valid cell; they read 0.
+template <int H>
+__device__ __forceinline__ double load_cell(const double *__restrict__ p, const StageArgs &a, int c, int i, int j,
+                                            int k) {
+    const bool inside = (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
+    if (!inside) {
+        if (i < -H || i >= a.nx + H || j < -H || j >= a.ny + H || k < -H || k >= a.nz + H) return 0.0;
+        if (a.ghost_mode == 0) {
+            bool phys = false;
+            if (i < 0 || i >= a.nx) {
+                if (a.periodic[0]) i += (i < 0) ? a.nx : -a.nx;
+                else phys = true;
+            }
+            if (j < 0 || j >= a.ny) {
+                if (a.periodic[1]) j += (j < 0) ? a.ny : -a.ny;
+                else phys = true;
+            }
+            if (k < 0 || k >= a.nz) {
+                if (a.periodic[2]) k += (k < 0) ? a.nz : -a.nz;
+                else phys = true;
+            }
+            if (phys) return a.dirichlet ? a.target[c] : 0.0;
+        }
+    }
+    return p[i + j * a.sj + k * a.sk + c * a.sc];
+}
+
+template <class Sys, int N, int R, int TY> struct FusedStage {
+    static constexpr int TX = 64;
+    static constexpr int H = N > R ? N : R;
+    static constexpr int NC = Sys::NC;
+    static constexpr int NS = Staging<Sys, R>::count();
+    static constexpr int LW = TX + 2 * H;
+    static constexpr int LH = TY + 2 * H;
+    static constexpr int NT = TX * TY;
+    static constexpr int NHALO = 2 * H * (TX + TY);
+    static constexpr int HPT = (NHALO + NT - 1) / NT;
+    static constexpr int Q = 2 * H + 1;
+
+    struct View {
+        const double (&q)[NS][Q];
+        const double (*lds)[LH][LW];
+        const double (&cen)[NC];
+        const StageArgs &a;
+        int lx, ly; // LDS coordinates of the centre cell
+        __device__ __forceinline__ double gf1(int ax) const { return a.gf1[ax]; }
+        __device__ __forceinline__ double gf2(int ax) const { return a.gf2[ax]; }
+        template <int C, int AX, int S> __device__ __forceinline__ double at() const {
+            if constexpr (S == 0) {
+                return cen[C];
+            } else {
+                constexpr int sl = Staging<Sys, R>::slot(C);
+                static_assert(Staging<Sys, R>::staged(C), "off-centre read of an unstaged component");
+                if constexpr (AX == 2) return q[sl][H + S];
+                else if constexpr (AX == 0) return lds[sl][ly][lx + S];
+                else return lds[sl][ly + S][lx];
+            }
+        }
+    };
+
+    __device__ __forceinline__ static void run(const StageArgs &a) {
+        __shared__ double lds[NS][LH][LW];
+
+        const int tid = threadIdx.x;
+        const int lane = tid & 63;
+        const int row = tid >> 6;
+
+        // block -> tile; optionally give each XCD a contiguous run of tiles
+        const int nb = gridDim.x;
+        int bid = blockIdx.x;
+        if (a.xcd_remap) {
+            const int per = nb / 8, rem = nb % 8, x = bid % 8;
+            bid = x * per + (x < rem ? x : rem) + bid / 8;
+        }
+        const int tix = bid % a.tiles_x;
+        const int t2 = bid / a.tiles_x;
+        const int tiy = t2 % a.tiles_y;
+        const int zc = t2 / a.tiles_y;
+        const int x0 = tix * TX, y0 = tiy * TY;
+        const int z0 = zc * a.zchunk;
+        const int z1 = min(z0 + a.zchunk, a.nz);
+        const int gi = x0 + lane, gj = y0 + row;
+        const bool active = gi < a.nx && gj < a.ny;
+
+        const int stage = a.stage;
+        const bool is_first = stage == kStage0 || stage == kEuler;
+        const bool write_argn = stage < kStage3;
+        const bool read_init = write_argn && !is_first;
+
+        // halo cells owned by this thread (LDS coordinates, -H based)
+        int hlx[HPT], hly[HPT];
+        bool hok[HPT];
+#pragma unroll
+        for (int m = 0; m < HPT; ++m) {
+            const int h = tid + m * NT;
+            hok[m] = h < NHALO;
+            if (h < 2 * H * TX) { // rows above/below the tile
+                const int r = h / TX;
+                hly[m] = r < H ? r - H : TY + (r - H);
+                hlx[m] = h % TX;
+            } else { // columns left/right of the tile
+                const int e = h - 2 * H * TX;
+                const int mm = e % (2 * H);
+                hly[m] = e / (2 * H);
+                hlx[m] = mm < H ? mm - H : TX + (mm - H);
+            }
+        }
+
+        // ABL damping rate of this column's cells depends on z too; the x/y
+        // part of the distance is fixed per lane.
+        int cxy2 = 0;
+        const int T = a.abl_T;
+        if (T > 0) {
+            const int dxv = gi - min(max(gi, T), a.nx - 1 - T);
+            const int dyv = gj - min(max(gj, T), a.ny - 1 - T);
+            cxy2 = dxv * dxv + dyv * dyv;
+        }
+
+        double q[NS][Q];
+        double hv[NS][HPT];
+
+        static_for<0, NC>([&]<int C>() {
+            if constexpr (Staging<Sys, R>::staged(C)) {
+                constexpr int sl = Staging<Sys, R>::slot(C);
+#pragma unroll
+                for (int m = 0; m < Q; ++m) q[sl][m] = load_cell<H>(a.src, a, C, gi, gj, z0 - H + m);
+#pragma unroll
+                for (int m = 0; m < HPT; ++m)
+                    hv[sl][m] = hok[m] ? load_cell<H>(a.src, a, C, x0 + hlx[m], y0 + hly[m], z0) : 0.0;
+            }
+        });
+
+        for (int z = z0; z < z1; ++z) {
+            // prefetch the next plane of the queue and of the halo
+            double qn[NS], hvn[NS][HPT];
+            const bool more = z + 1 < z1;
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (Staging<Sys, R>::staged(C)) {
+                    constexpr int sl = Staging<Sys, R>::slot(C);
+                    qn[sl] = more ? load_cell<H>(a.src, a, C, gi, gj, z + 1 + H) : 0.0;
+#pragma unroll
+                    for (int m = 0; m < HPT; ++m)
+                        hvn[sl][m] =
+                            (more && hok[m]) ? load_cell<H>(a.src, a, C, x0 + hlx[m], y0 + hly[m], z + 1) : 0.0;
+                }
+            });
+
+            // centre values of this cell: source, init and old acc
+            const int off = gi + gj * a.sj + z * a.sk;
+            double cen[NC], initc[NC], accc[NC];
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (Staging<Sys, R>::staged(C)) {
+                    cen[C] = q[Staging<Sys, R>::slot(C)][H];
+                } else {
+                    cen[C] = active ? a.src[off + C * a.sc] : 0.0;
+                }
+                initc[C] = (active && read_init) ? a.init[off + C * a.sc] : cen[C];
+                accc[C] = (active && !is_first) ? a.acc[off + C * a.sc] : 0.0;
+            });
+
+            __syncthreads(); // previous plane's LDS reads are done
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                lds[s][row + H][lane + H] = q[s][H];
+#pragma unroll
+                for (int m = 0; m < HPT; ++m)
+                    if (hok[m]) lds[s][hly[m] + H][hlx[m] + H] = hv[s][m];
+            }
+            __syncthreads();
+
+            if (active) {
+                double sigma = 0.0;
+                if (T > 0) {
+                    const int dzv = z - min(max(z, T), a.nz - 1 - T);
+                    const int d2 = cxy2 + dzv * dzv;
+                    sigma = d2 == 0 ? 0.0 : a.sigma_tab[d2];
+                }
+                const View v{q, lds, cen, a, lane + H, row + H};
+                static_for<0, NC>([&]<int C>() {
+                    double k = Sys::template user<N, C>(v, a);
+                    if constexpr (R > 0) {
+                        double ko_term = 0.0;
+                        ko_term += KO<R, 0, C>(v);
+                        ko_term += KO<R, 1, C>(v);
+                        ko_term += KO<R, 2, C>(v);
+                        k = k + a.ko_strength * ko_term;
+                    }
+                    if (T > 0) k = k - sigma * (cen[C] - a.target[C]);
+                    if (is_first) a.acc[off + C * a.sc] = initc[C] + a.b * k;
+                    else a.acc[off + C * a.sc] = accc[C] + a.b * k;
+                    if (write_argn) a.argn[off + C * a.sc] = initc[C] + a.a_next * k;
+                });
+            }
+
+            // advance the queue
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+#pragma unroll
+                for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
+                q[s][Q - 1] = qn[s];
+#pragma unroll
+                for (int m = 0; m < HPT; ++m) hv[s][m] = hvn[s][m];
+            }
+        }
+    }
+};
+
+template <class Sys, int N, int R, int TY>
+__global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
+    FusedStage<Sys, N, R, TY>::run(a);
+}
+
+template <class Sys, int N, int R> hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
+    StageArgs a = a0;
+    a.tiles_x = (a.nx + 63) / 64;
+    a.tiles_y = (a.ny + tile_y - 1) / tile_y;
+    a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
+    const long long nblocks = (long long)a.tiles_x * a.tiles_y * a.nzc;
+    if (nblocks <= 0 || nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)nblocks);
+    switch (tile_y) {
+    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2>), grid, dim3(128), 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4>), grid, dim3(256), 0, stream, a); break;
+    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8>), grid, dim3(512), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Supported (stencil width, KO order) pairs: r in {0, n, n+1} with r <= 4.
+template <class Sys> LaunchFn lookup_fused(int n, int r) {
+#define TSP_CASE(NN, RR)                                                                                              \
+    if (n == NN && r == RR) return &launch_fused<Sys, NN, RR>;
+    TSP_CASE(1, 0)
+    TSP_CASE(1, 1)
+    TSP_CASE(1, 2)
+    TSP_CASE(2, 0)
+    TSP_CASE(2, 2)
+    TSP_CASE(2, 3)
+    TSP_CASE(3, 0)
+    TSP_CASE(3, 3)
+    TSP_CASE(3, 4)
+    TSP_CASE(4, 0)
+    TSP_CASE(4, 4)
+#undef TSP_CASE
+    return nullptr;
+}
+
+} // namespace tsp

@@ -1,0 +1,156 @@
+"""Host-side solver object over the C ABI.
+
+``Solver`` plays the role of ``treesap::SingleLevelRK4Integrator`` /
+``SingleLevelEulerIntegrator`` (integrators/SingleLevelRK4Integrator.hpp:39-167,
+integrators/SingleLevelEulerIntegrator.hpp:31-89) for the registered systems:
+construct from a model description, upload the initial condition (the
+``initialise()`` step, SingleLevelIntegrator.hpp:75-93), then ``step()``.
+Unlike the reference it exposes the state (``get_state``) so results can be
+checked. All compute runs in the HIP library; this module only moves arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .workloads import Case, make_desc
+
+
+def _is_torch_cuda(x) -> bool:
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+class Solver:
+    def __init__(self, case: Case, stream=None):
+        self.case = case
+        self._desc = make_desc(case)
+        handle = C.c_void_p()
+        st = _lib.lib().tsp_create(C.byref(self._desc), C.c_void_p(stream), C.byref(handle))
+        _lib.check(st, "tsp_create")
+        self._h = handle
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().tsp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, state) -> None:
+        """initialise(): upload the valid-region state (ncomp, nz, ny, nx) float64."""
+        if _is_torch_cuda(state):
+            if tuple(state.shape) != self.case.shape or str(state.dtype) != "torch.float64":
+                raise ValueError(f"state must be float64 of shape {self.case.shape}")
+            state = state.contiguous()
+            _lib.check(_lib.lib().tsp_set_state(self._h, C.c_void_p(state.data_ptr()), 1), "tsp_set_state")
+            return
+        arr = np.ascontiguousarray(state, dtype=np.float64)
+        if arr.shape != self.case.shape:
+            raise ValueError(f"state must have shape {self.case.shape}, got {arr.shape}")
+        _lib.check(_lib.lib().tsp_set_state(self._h, arr.ctypes.data_as(C.c_void_p), 0), "tsp_set_state")
+
+    def get_state(self, out=None):
+        """state_old (valid region) as a numpy array, or into a float64 torch CUDA tensor."""
+        if out is not None and _is_torch_cuda(out):
+            if not out.is_contiguous() or tuple(out.shape) != self.case.shape:
+                raise ValueError("out must be contiguous with the state shape")
+            _lib.check(_lib.lib().tsp_get_state(self._h, C.c_void_p(out.data_ptr()), 1), "tsp_get_state")
+            return out
+        arr = np.empty(self.case.shape, dtype=np.float64)
+        _lib.check(_lib.lib().tsp_get_state(self._h, arr.ctypes.data_as(C.c_void_p), 0), "tsp_get_state")
+        return arr
+
+    def get_register(self, role: int) -> np.ndarray:
+        """Whole ghosted register (ncomp, nz+2G, ny+2G, nx+2G)."""
+        G = self.case.ghost
+        n = self.case.n
+        arr = np.empty((self.case.ncomp, n[2] + 2 * G, n[1] + 2 * G, n[0] + 2 * G), dtype=np.float64)
+        _lib.check(_lib.lib().tsp_get_register(self._h, role, arr.ctypes.data_as(C.c_void_p)), "tsp_get_register")
+        return arr
+
+    def fill_ghosts(self, role: int = _lib.ROLE_OLD) -> None:
+        _lib.check(_lib.lib().tsp_fill_ghosts(self._h, role), "tsp_fill_ghosts")
+
+    # -- stepping ----------------------------------------------------------
+    def rk4_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_rk4_steps(self._h, int(nsteps)), "tsp_rk4_steps")
+
+    def euler_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_euler_steps(self._h, int(nsteps)), "tsp_euler_steps")
+
+    def graph_rk4_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_graph_rk4_steps(self._h, int(nsteps)), "tsp_graph_rk4_steps")
+
+    def step(self) -> None:
+        """Integrator::step() with the RK4 scheme."""
+        self.rk4_steps(1)
+
+    def sync(self) -> None:
+        _lib.check(_lib.lib().tsp_sync(self._h), "tsp_sync")
+
+    @property
+    def time(self) -> float:
+        t = C.c_double()
+        s = C.c_int64()
+        _lib.check(_lib.lib().tsp_get_time(self._h, C.byref(t), C.byref(s)), "tsp_get_time")
+        return t.value
+
+    @property
+    def steps(self) -> int:
+        t = C.c_double()
+        s = C.c_int64()
+        _lib.check(_lib.lib().tsp_get_time(self._h, C.byref(t), C.byref(s)), "tsp_get_time")
+        return s.value
+
+    # -- kernel configuration ---------------------------------------------
+    @property
+    def kernel(self) -> dict:
+        cfg = _lib.TspKernelCfg()
+        _lib.check(_lib.lib().tsp_get_kernel(self._h, C.byref(cfg)), "tsp_get_kernel")
+        return cfg.as_dict()
+
+    @kernel.setter
+    def kernel(self, cfg: dict) -> None:
+        cur = self.kernel
+        cur.update(cfg)
+        c = _lib.TspKernelCfg(**cur)
+        _lib.check(_lib.lib().tsp_set_kernel(self._h, C.byref(c)), "tsp_set_kernel")
+
+    def autotune(self, steps: int = 2) -> dict:
+        cfg = _lib.TspKernelCfg()
+        _lib.check(_lib.lib().tsp_autotune(self._h, int(steps), C.byref(cfg)), "tsp_autotune")
+        return cfg.as_dict()
+
+    def profile_rk4(self, nsteps: int):
+        """Eager steps with HIP events around each launch: (per-stage ms[4], ghost-fill ms)."""
+        stage = (C.c_double * 4)()
+        fill = C.c_double()
+        _lib.check(_lib.lib().tsp_profile_rk4(self._h, int(nsteps), stage, C.byref(fill)), "tsp_profile_rk4")
+        return list(stage), fill.value
+
+    def time_rk4(self, nsteps: int, use_graph: bool = True) -> float:
+        """Device time (ms) of `nsteps` RK4 steps between two HIP events on the solver's stream."""
+        ms = C.c_double()
+        _lib.check(_lib.lib().tsp_time_rk4(self._h, int(nsteps), int(bool(use_graph)), C.byref(ms)), "tsp_time_rk4")
+        return ms.value
+
+    def rk4_bytes_per_cell(self) -> int:
+        return int(_lib.lib().tsp_rk4_bytes_per_cell(C.byref(self._desc)))
