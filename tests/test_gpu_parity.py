@@ -72,7 +72,7 @@ CASES = [
 def test_rk4_matches_oracle(system, n, sn, ko, per, abl, ghost_mode):
     _say(f"[parity] rk4 {system} n={n} stencil={sn} ko={ko} periodic={per} abl={abl} ghost_mode={ghost_mode}")
     case, state = small_case(system, n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl,
-                             gamma=0.3 if system == "nonlinear" else 0.0)
+                             gamma=0.05 if system == "nonlinear" else 0.0)
     got, want = run_both(case, state, 7, kernel={"ghost_mode": ghost_mode})
     err = rel_l2_per_comp(got, want)
     assert np.all(err <= REL_TOL), err

@@ -102,10 +102,28 @@ PROTOTYPES = {
 _lib = None
 
 
+def _share_torch_runtime() -> None:
+    """Load PyTorch's HIP runtime before the library, when PyTorch is installed.
+
+    PyTorch's ROCm libraries depend on ``libamdhip64.so`` found next to them
+    (RPATH $ORIGIN), whose soname is ``libamdhip64.so.7``; this library depends
+    on ``libamdhip64.so.7``. Loading PyTorch first makes the dynamic loader
+    resolve our dependency to the runtime that is already mapped, so the
+    process holds one HIP runtime: device pointers, streams and
+    ``torch.cuda.synchronize()`` are then shared with PyTorch. Loaded the other
+    way round the process would map two HIP/HSA runtimes.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> C.CDLL:
     """Load the HIP library (once). Raises if it is missing: there is no fallback."""
     global _lib
     if _lib is None:
+        _share_torch_runtime()
         path = Path(os.environ.get("TSP_LIBRARY", LIB_PATH))
         if not path.exists():
             raise RuntimeError(

@@ -28,6 +28,7 @@ ARCH = os.environ.get("TSP_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}",
+    "-mcode-object-version=5",
     "-O3",
     "-std=c++20",
     "-fPIC",

@@ -119,6 +119,8 @@ struct StageArgs {
     int32_t tiles_x, tiles_y, nzc;
     int32_t xcd_remap;
     double b, a_next;        // dt*b_s and dt*a_{s+1} (integrators/SingleLevelRK4Integrator.hpp:119-121)
+    double c_init_argn;      // adjoint only: coefficient of lambda' in the next stage argument
+    double src_scale;        // adjoint only: factor applied to every source value on load
     double gf1[3];           // 1/dx   (stencil_impl.hpp:213, stencil.hpp:94)
     double gf2[3];           // 1/(dx*dx) (stencil_impl.hpp:255)
     double ko_strength;
@@ -131,6 +133,7 @@ struct StageArgs {
 using LaunchFn = hipError_t (*)(const StageArgs &a, int tile_y, hipStream_t stream);
 
 LaunchFn lookup_phipi(int n, int r);
+LaunchFn lookup_phipi_adjoint(int n, int r);
 LaunchFn lookup_phiu(int n, int r);
 LaunchFn lookup_nonlinear(int n, int r);
 

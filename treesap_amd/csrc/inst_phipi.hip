@@ -6,3 +6,7 @@
 namespace tsp {
 LaunchFn lookup_phipi(int n, int r) { return lookup_fused<SysPhiPi>(n, r); }
 } // namespace tsp
+
+namespace tsp {
+LaunchFn lookup_phipi_adjoint(int n, int r) { return lookup_fused<SysPhiPiT, true>(n, r); }
+} // namespace tsp

@@ -127,6 +127,27 @@ struct SysNonLinear {
     }
 };
 
+// Transpose of the linear (phi, Pi) operator, for the discrete adjoint. With
+// homogeneous ghosts (zero Dirichlet values or periodic images) D_aa and the
+// KO operator are symmetric and the ABL term is diagonal, so
+//   [[KO - s, I], [Lap, KO - s]]^T = [[KO - s, Lap], [I, KO - s]]:
+//   phibar_t = Lap(Pibar) (+ KO, ABL),  Pibar_t = phibar (+ KO, ABL).
+struct SysPhiPiT {
+    static constexpr int NC = 2;
+    static constexpr bool user_stencil(int c) { return c == 1; }
+    template <int N, int C, class V> __device__ __forceinline__ static double user(const V &v, const StageArgs &) {
+        if constexpr (C == 0) {
+            double result = 0.0;
+            result += D2<N, 0, 1>(v);
+            result += D2<N, 1, 1>(v);
+            result += D2<N, 2, 1>(v);
+            return result;
+        } else {
+            return v.template at<0, 0, 0>();
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // kernel
 // ---------------------------------------------------------------------------
@@ -172,7 +193,7 @@ __device__ __forceinline__ double load_cell(const double *__restrict__ p, const 
     return p[i + j * a.sj + k * a.sk + c * a.sc];
 }
 
-template <class Sys, int N, int R, int TY> struct FusedStage {
+template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     static constexpr int TX = 64;
     static constexpr int H = N > R ? N : R;
     static constexpr int NC = Sys::NC;
@@ -183,10 +204,13 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
     static constexpr int NHALO = 2 * H * (TX + TY);
     static constexpr int HPT = (NHALO + NT - 1) / NT;
     static constexpr int Q = 2 * H + 1;
+    static constexpr int LDS_DOUBLES = NS * LH * LW;
+
+    using Plane = double[LH][LW];
 
     struct View {
         const double (&q)[NS][Q];
-        const double (*lds)[LH][LW];
+        const Plane *lds;
         const double (&cen)[NC];
         const StageArgs &a;
         int lx, ly; // LDS coordinates of the centre cell
@@ -205,34 +229,41 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
         }
     };
 
-    __device__ __forceinline__ static void run(const StageArgs &a) {
-        __shared__ double lds[NS][LH][LW];
+    // Source value as the stage sees it (the adjoint's first reverse stage
+    // reads b_3 * lambda' without materialising it).
+    __device__ __forceinline__ static double src_at(const StageArgs &a, int c, int i, int j, int k) {
+        const double v = load_cell<H>(a.src, a, c, i, j, k);
+        if constexpr (ADJ) return a.src_scale * v;
+        else return v;
+    }
 
+    // One 64 x TY column tile over z planes [z0, z1). `lds_raw` holds
+    // LDS_DOUBLES doubles of LDS. Every thread of the block must call this.
+    //
+    // Forward (RK4 stage s / Euler, integrators/SingleLevelRK4Integrator.hpp:142-154):
+    //   k = F(src);  acc = (first ? init : acc) + b*k;  argn = init + a_next*k
+    // Adjoint (reverse stage j of the discrete adjoint RK4, Sys = transpose system):
+    //   nu = L^T mu;  acc = (first ? lambda' : acc) + nu;  argn = b_{j-1}*lambda' + a_j*nu
+    __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
+                                                    double *lds_raw) {
+        Plane *lds = reinterpret_cast<Plane *>(lds_raw);
         const int tid = threadIdx.x;
         const int lane = tid & 63;
         const int row = tid >> 6;
-
-        // block -> tile; optionally give each XCD a contiguous run of tiles
-        const int nb = gridDim.x;
-        int bid = blockIdx.x;
-        if (a.xcd_remap) {
-            const int per = nb / 8, rem = nb % 8, x = bid % 8;
-            bid = x * per + (x < rem ? x : rem) + bid / 8;
-        }
-        const int tix = bid % a.tiles_x;
-        const int t2 = bid / a.tiles_x;
-        const int tiy = t2 % a.tiles_y;
-        const int zc = t2 / a.tiles_y;
-        const int x0 = tix * TX, y0 = tiy * TY;
-        const int z0 = zc * a.zchunk;
-        const int z1 = min(z0 + a.zchunk, a.nz);
         const int gi = x0 + lane, gj = y0 + row;
         const bool active = gi < a.nx && gj < a.ny;
 
         const int stage = a.stage;
-        const bool is_first = stage == kStage0 || stage == kEuler;
-        const bool write_argn = stage < kStage3;
-        const bool read_init = write_argn && !is_first;
+        bool is_first, write_argn, read_init;
+        if constexpr (ADJ) {
+            is_first = stage == kStage3;
+            write_argn = stage > kStage0;
+            read_init = write_argn || is_first;
+        } else {
+            is_first = stage == kStage0 || stage == kEuler;
+            write_argn = stage < kStage3;
+            read_init = write_argn && !is_first;
+        }
 
         // halo cells owned by this thread (LDS coordinates, -H based)
         int hlx[HPT], hly[HPT];
@@ -253,8 +284,7 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
             }
         }
 
-        // ABL damping rate of this column's cells depends on z too; the x/y
-        // part of the distance is fixed per lane.
+        // ABL damping: the x/y part of the squared distance to the bulk box is fixed per lane
         int cxy2 = 0;
         const int T = a.abl_T;
         if (T > 0) {
@@ -270,10 +300,9 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
             if constexpr (Staging<Sys, R>::staged(C)) {
                 constexpr int sl = Staging<Sys, R>::slot(C);
 #pragma unroll
-                for (int m = 0; m < Q; ++m) q[sl][m] = load_cell<H>(a.src, a, C, gi, gj, z0 - H + m);
+                for (int m = 0; m < Q; ++m) q[sl][m] = src_at(a, C, gi, gj, z0 - H + m);
 #pragma unroll
-                for (int m = 0; m < HPT; ++m)
-                    hv[sl][m] = hok[m] ? load_cell<H>(a.src, a, C, x0 + hlx[m], y0 + hly[m], z0) : 0.0;
+                for (int m = 0; m < HPT; ++m) hv[sl][m] = hok[m] ? src_at(a, C, x0 + hlx[m], y0 + hly[m], z0) : 0.0;
             }
         });
 
@@ -284,11 +313,10 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
             static_for<0, NC>([&]<int C>() {
                 if constexpr (Staging<Sys, R>::staged(C)) {
                     constexpr int sl = Staging<Sys, R>::slot(C);
-                    qn[sl] = more ? load_cell<H>(a.src, a, C, gi, gj, z + 1 + H) : 0.0;
+                    qn[sl] = more ? src_at(a, C, gi, gj, z + 1 + H) : 0.0;
 #pragma unroll
                     for (int m = 0; m < HPT; ++m)
-                        hvn[sl][m] =
-                            (more && hok[m]) ? load_cell<H>(a.src, a, C, x0 + hlx[m], y0 + hly[m], z + 1) : 0.0;
+                        hvn[sl][m] = (more && hok[m]) ? src_at(a, C, x0 + hlx[m], y0 + hly[m], z + 1) : 0.0;
                 }
             });
 
@@ -299,7 +327,9 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
                 if constexpr (Staging<Sys, R>::staged(C)) {
                     cen[C] = q[Staging<Sys, R>::slot(C)][H];
                 } else {
-                    cen[C] = active ? a.src[off + C * a.sc] : 0.0;
+                    double v = active ? a.src[off + C * a.sc] : 0.0;
+                    if constexpr (ADJ) v = a.src_scale * v;
+                    cen[C] = v;
                 }
                 initc[C] = (active && read_init) ? a.init[off + C * a.sc] : cen[C];
                 accc[C] = (active && !is_first) ? a.acc[off + C * a.sc] : 0.0;
@@ -333,9 +363,14 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
                         k = k + a.ko_strength * ko_term;
                     }
                     if (T > 0) k = k - sigma * (cen[C] - a.target[C]);
-                    if (is_first) a.acc[off + C * a.sc] = initc[C] + a.b * k;
-                    else a.acc[off + C * a.sc] = accc[C] + a.b * k;
-                    if (write_argn) a.argn[off + C * a.sc] = initc[C] + a.a_next * k;
+                    if constexpr (ADJ) {
+                        a.acc[off + C * a.sc] = (is_first ? initc[C] : accc[C]) + k;
+                        if (write_argn) a.argn[off + C * a.sc] = a.c_init_argn * initc[C] + a.a_next * k;
+                    } else {
+                        if (is_first) a.acc[off + C * a.sc] = initc[C] + a.b * k;
+                        else a.acc[off + C * a.sc] = accc[C] + a.b * k;
+                        if (write_argn) a.argn[off + C * a.sc] = initc[C] + a.a_next * k;
+                    }
                 });
             }
 
@@ -350,14 +385,33 @@ template <class Sys, int N, int R, int TY> struct FusedStage {
             }
         }
     }
+
+    // single box: block -> (tile x, tile y, z chunk), optionally giving each
+    // XCD (blocks are dealt to the 8 XCDs round robin) a contiguous run of tiles
+    __device__ __forceinline__ static void run(const StageArgs &a) {
+        __shared__ double lds[LDS_DOUBLES];
+        const int nb = gridDim.x;
+        int bid = blockIdx.x;
+        if (a.xcd_remap) {
+            const int per = nb / 8, rem = nb % 8, x = bid % 8;
+            bid = x * per + (x < rem ? x : rem) + bid / 8;
+        }
+        const int tix = bid % a.tiles_x;
+        const int t2 = bid / a.tiles_x;
+        const int tiy = t2 % a.tiles_y;
+        const int zc = t2 / a.tiles_y;
+        const int z0 = zc * a.zchunk;
+        run_tile(a, tix * TX, tiy * TY, z0, min(z0 + a.zchunk, a.nz), lds);
+    }
 };
 
-template <class Sys, int N, int R, int TY>
+template <class Sys, int N, int R, int TY, bool ADJ>
 __global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
-    FusedStage<Sys, N, R, TY>::run(a);
+    FusedStage<Sys, N, R, TY, ADJ>::run(a);
 }
 
-template <class Sys, int N, int R> hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
+template <class Sys, int N, int R, bool ADJ = false>
+hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     StageArgs a = a0;
     a.tiles_x = (a.nx + 63) / 64;
     a.tiles_y = (a.ny + tile_y - 1) / tile_y;
@@ -366,18 +420,18 @@ template <class Sys, int N, int R> hipError_t launch_fused(const StageArgs &a0, 
     if (nblocks <= 0 || nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)nblocks);
     switch (tile_y) {
-    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2>), grid, dim3(128), 0, stream, a); break;
-    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4>), grid, dim3(256), 0, stream, a); break;
-    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8>), grid, dim3(512), 0, stream, a); break;
+    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), 0, stream, a); break;
+    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ>), grid, dim3(512), 0, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 // Supported (stencil width, KO order) pairs: r in {0, n, n+1} with r <= 4.
-template <class Sys> LaunchFn lookup_fused(int n, int r) {
+template <class Sys, bool ADJ = false> LaunchFn lookup_fused(int n, int r) {
 #define TSP_CASE(NN, RR)                                                                                              \
-    if (n == NN && r == RR) return &launch_fused<Sys, NN, RR>;
+    if (n == NN && r == RR) return &launch_fused<Sys, NN, RR, ADJ>;
     TSP_CASE(1, 0)
     TSP_CASE(1, 1)
     TSP_CASE(1, 2)
