@@ -82,6 +82,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle stepping")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--kernel", default=None,
+                    help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt] (skips the autotuner)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,7 +105,10 @@ def main() -> int:
 
     s = Solver(case)
     s.set_state(state)
-    if not args.no_tune:
+    if args.kernel:
+        vals = [int(v) for v in args.kernel.split(",")]
+        s.kernel = dict(zip(("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt"), vals))
+    elif not args.no_tune:
         t0 = time.perf_counter()
         cfg = s.autotune(3)
         log(f"[bench] autotune chose {cfg} in {time.perf_counter() - t0:.1f} s")
@@ -118,6 +125,15 @@ def main() -> int:
     if not np.isfinite(out).all():
         log("[bench] non-finite state after the timed steps")
         return 1
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB x3 done")
 
     # per-stage kernel durations (eager launches, events around each launch)
     stage_ms, fill_ms = s.profile_rk4(10)

@@ -92,9 +92,10 @@ typedef struct tsp_desc {
  * configuration produces bit-identical results; only the speed differs. */
 typedef struct tsp_kernel_cfg {
     int32_t ghost_mode; /* 0: virtual ghosts (no fill pass); 1: FillPatch pass then plain reads */
-    int32_t tile_y;     /* rows (waves) per workgroup: 2, 4 or 8; tile_x is 64 */
+    int32_t tile_y;     /* rows (waves) per workgroup: 2, 4, 8 or 16; tile_x is 64 */
     int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
     int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
+    int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together) */
 } tsp_kernel_cfg;
 
 typedef struct tsp_solver tsp_solver;

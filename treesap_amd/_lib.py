@@ -58,6 +58,7 @@ class TspKernelCfg(C.Structure):
         ("tile_y", C.c_int32),
         ("zchunk", C.c_int32),
         ("xcd_remap", C.c_int32),
+        ("zalt", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

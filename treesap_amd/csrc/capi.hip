@@ -89,7 +89,7 @@ struct tsp_solver {
     bool own_stream = false;
     double *sigma_tab = nullptr;
     tsp::LaunchFn launch = nullptr;
-    tsp_kernel_cfg cfg{0, 4, 64, 1};
+    tsp_kernel_cfg cfg{0, 8, 64, 1, 1};
     double time = 0.0;
     int64_t steps = 0;
     double gf1[3]{}, gf2[3]{};
@@ -148,9 +148,11 @@ tsp_status validate(const tsp_desc *d) {
 tsp_status validate_cfg(const tsp_kernel_cfg *c) {
     if (!c) return fail(TSP_INVALID_ARG, "kernel config is NULL");
     if (c->ghost_mode != 0 && c->ghost_mode != 1) return fail(TSP_INVALID_ARG, "ghost_mode must be 0 or 1");
-    if (c->tile_y != 2 && c->tile_y != 4 && c->tile_y != 8) return fail(TSP_INVALID_ARG, "tile_y must be 2, 4 or 8");
+    if (c->tile_y != 2 && c->tile_y != 4 && c->tile_y != 8 && c->tile_y != 16)
+        return fail(TSP_INVALID_ARG, "tile_y must be 2, 4, 8 or 16");
     if (c->zchunk < 1) return fail(TSP_INVALID_ARG, "zchunk must be >= 1");
     if (c->xcd_remap != 0 && c->xcd_remap != 1) return fail(TSP_INVALID_ARG, "xcd_remap must be 0 or 1");
+    if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
     return TSP_OK;
 }
 
@@ -190,6 +192,7 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     a.nx = s->d.n[0];
     a.ny = s->d.n[1];
     a.nz = s->d.n[2];
+    a.G = s->G;
     a.sj = (int32_t)s->sj;
     a.sk = (int32_t)s->sk;
     a.sc = (int32_t)s->sc;
@@ -200,6 +203,7 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     a.abl_T = s->d.abl_thickness;
     a.zchunk = s->cfg.zchunk;
     a.xcd_remap = s->cfg.xcd_remap;
+    a.zalt = s->cfg.zalt;
     a.b = b;
     a.a_next = a_next;
     for (int i = 0; i < 3; ++i) {
@@ -343,6 +347,13 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc"));
         e = hipMemsetAsync(s->base[r], 0, sizeof(double) * s->reg_elems, s->stream);
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemsetAsync"));
+    }
+    // Ghost cells outside a non-periodic face only ever hold the boundary
+    // values (stages write valid cells only), so they are written once here
+    // for every register; the virtual-ghost stage kernels read them as is.
+    for (int r = 0; r < 4; ++r) {
+        hipError_t e = tsp::launch_ghost_fill(ghost_args(s, r), s->stream);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "ghost prefill"));
     }
     if (d->abl_thickness > 0) {
         // ABLDecorator.hpp:115-132 tabulated by squared integer distance to the bulk box
@@ -637,7 +648,8 @@ bool cache_lookup(const std::string &key, tsp_kernel_cfg *cfg) {
         char k[256];
         tsp_kernel_cfg c{};
         double ms = 0;
-        if (std::sscanf(line, "%255s %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap, &ms) == 6 &&
+        if (std::sscanf(line, "%255s %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap, &c.zalt,
+                        &ms) == 7 &&
             key == k && validate_cfg(&c) == TSP_OK) {
             *cfg = c; // the last matching line wins
             found = true;
@@ -652,7 +664,7 @@ void cache_store(const std::string &key, const tsp_kernel_cfg &c, double ms) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap, ms);
+    std::fprintf(f, "%s %d %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap, c.zalt, ms);
     std::fclose(f);
 }
 
@@ -689,32 +701,42 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     for (int z : {8, 16, 32, 64, 128, 256})
         if (z < s->d.n[2]) zchunks.push_back(z);
     zchunks.push_back(s->d.n[2]);
+    // candidates: every tile height, z chunk, XCD mapping and z-march order
+    // with virtual ghosts, then the best geometry with the fill-pass mode
+    std::vector<tsp_kernel_cfg> cands;
+    for (int ty : {2, 4, 8, 16})
+        for (int zc : zchunks)
+            for (int xr = 0; xr < 2; ++xr)
+                for (int za = 0; za < 2; ++za) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za});
     double best_ms = 1e300;
-    for (int gm = 0; gm < 2 && st == TSP_OK; ++gm)
-        for (int ty : {2, 4, 8})
-            for (int zc : zchunks)
-                for (int xr = 0; xr < 2 && st == TSP_OK; ++xr) {
-                    s->cfg = tsp_kernel_cfg{gm, ty, zc, xr};
-                    st = enqueue_rk4_step(s, nullptr); // warm-up
-                    if (st != TSP_OK) break;
-                    if ((e = hipEventRecord(e0, s->stream)) != hipSuccess) { st = hip_fail(e, "hipEventRecord"); break; }
-                    for (int n = 0; n < steps && st == TSP_OK; ++n) st = enqueue_rk4_step(s, nullptr);
-                    if (st != TSP_OK) break;
-                    if ((e = hipEventRecord(e1, s->stream)) != hipSuccess) { st = hip_fail(e, "hipEventRecord"); break; }
-                    if ((e = hipEventSynchronize(e1)) != hipSuccess) { st = hip_fail(e, "hipEventSynchronize"); break; }
-                    float ms = 0.f;
-                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) { st = hip_fail(e, "hipEventElapsedTime"); break; }
-                    const double per = ms / steps;
-                    if (per < best_ms) {
-                        best_ms = per;
-                        best = s->cfg;
-                    }
-                    // keep the state bounded: restart every candidate from the saved state
-                    std::memcpy(s->role, saved_role, sizeof(saved_role));
-                    e = hipMemcpyAsync(s->base[saved_role[TSP_ROLE_OLD]], backup, sizeof(double) * s->reg_elems,
-                                       hipMemcpyDeviceToDevice, s->stream);
-                    if (e != hipSuccess) st = hip_fail(e, "autotune restore");
-                }
+    auto time_cfg = [&](const tsp_kernel_cfg &c) -> tsp_status {
+        s->cfg = c;
+        tsp_status r = enqueue_rk4_step(s, nullptr); // warm-up
+        if (r != TSP_OK) return r;
+        if ((e = hipEventRecord(e0, s->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        for (int n = 0; n < steps; ++n)
+            if ((r = enqueue_rk4_step(s, nullptr)) != TSP_OK) return r;
+        if ((e = hipEventRecord(e1, s->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+        const double per = ms / steps;
+        if (per < best_ms) {
+            best_ms = per;
+            best = c;
+        }
+        // restart every candidate from the saved state so values stay bounded
+        std::memcpy(s->role, saved_role, sizeof(saved_role));
+        e = hipMemcpyAsync(s->base[saved_role[TSP_ROLE_OLD]], backup, sizeof(double) * s->reg_elems,
+                           hipMemcpyDeviceToDevice, s->stream);
+        return e == hipSuccess ? TSP_OK : hip_fail(e, "autotune restore");
+    };
+    for (size_t i = 0; i < cands.size() && st == TSP_OK; ++i) st = time_cfg(cands[i]);
+    if (st == TSP_OK) {
+        tsp_kernel_cfg fill = best;
+        fill.ghost_mode = 1;
+        st = time_cfg(fill);
+    }
     // restore
     std::memcpy(s->role, saved_role, sizeof(saved_role));
     e = hipMemcpyAsync(s->base[saved_role[TSP_ROLE_OLD]], backup, sizeof(double) * s->reg_elems,

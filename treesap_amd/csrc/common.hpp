@@ -109,6 +109,7 @@ struct StageArgs {
     double *argn;            // arg_next
     const double *sigma_tab; // ABL sigma by squared distance to the bulk box
     int32_t nx, ny, nz;
+    int32_t G;               // ghost width of the allocation
     int32_t sj, sk, sc;      // element strides of the register layout
     int32_t stage;           // StageKind
     int32_t ghost_mode;      // 0 virtual ghosts, 1 read ghost memory
@@ -118,6 +119,7 @@ struct StageArgs {
     int32_t zchunk;
     int32_t tiles_x, tiles_y, nzc;
     int32_t xcd_remap;
+    int32_t zalt;            // alternate the z-march direction of consecutive chunks
     double b, a_next;        // dt*b_s and dt*a_{s+1} (integrators/SingleLevelRK4Integrator.hpp:119-121)
     double c_init_argn;      // adjoint only: coefficient of lambda' in the next stage argument
     double src_scale;        // adjoint only: factor applied to every source value on load

@@ -165,32 +165,19 @@ template <class Sys, int R> struct Staging {
     }
 };
 
-// Load element (i,j,k,c) of a register, mapping ghost indices. Indices more
-// than H outside the domain are never consumed by a valid cell; they read 0.
-template <int H>
-__device__ __forceinline__ double load_cell(const double *__restrict__ p, const StageArgs &a, int c, int i, int j,
-                                            int k) {
-    const bool inside = (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
-    if (!inside) {
-        if (i < -H || i >= a.nx + H || j < -H || j >= a.ny + H || k < -H || k >= a.nz + H) return 0.0;
-        if (a.ghost_mode == 0) {
-            bool phys = false;
-            if (i < 0 || i >= a.nx) {
-                if (a.periodic[0]) i += (i < 0) ? a.nx : -a.nx;
-                else phys = true;
-            }
-            if (j < 0 || j >= a.ny) {
-                if (a.periodic[1]) j += (j < 0) ? a.ny : -a.ny;
-                else phys = true;
-            }
-            if (k < 0 || k >= a.nz) {
-                if (a.periodic[2]) k += (k < 0) ? a.nz : -a.nz;
-                else phys = true;
-            }
-            if (phys) return a.dirichlet ? a.target[c] : 0.0;
-        }
+// Index map of one axis. Ghost cells along non-periodic axes hold their
+// boundary values permanently (they are written when the solver is created
+// and never touched by a stage), so in the virtual-ghost mode only periodic
+// axes are remapped: an index within H of the domain wraps once (H <= n). In
+// the fill-pass mode ghost memory is read as is. Indices that no valid cell
+// consumes (more than H outside; partial tiles) are clamped into the
+// allocation, their values are never used.
+__device__ __forceinline__ int map_axis(int i, int n, int wrap, int G) {
+    if (i < 0 || i >= n) {
+        if (wrap) i += i < 0 ? n : -n;
+        i = min(max(i, -G), n + G - 1);
     }
-    return p[i + j * a.sj + k * a.sk + c * a.sc];
+    return i;
 }
 
 template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
@@ -204,11 +191,14 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     static constexpr int NHALO = 2 * H * (TX + TY);
     static constexpr int HPT = (NHALO + NT - 1) / NT;
     static constexpr int Q = 2 * H + 1;
-    static constexpr int LDS_DOUBLES = NS * LH * LW;
+    static constexpr int LDS_PLANE = NS * LH * LW;
+    static constexpr int LDS_DOUBLES = 2 * LDS_PLANE; // double-buffered plane
 
     using Plane = double[LH][LW];
 
-    struct View {
+    // DOWN: the tile marches towards decreasing z; the queue then holds
+    // q[m] = plane z + H - m instead of z - H + m.
+    template <bool DOWN> struct View {
         const double (&q)[NS][Q];
         const Plane *lds;
         const double (&cen)[NC];
@@ -222,17 +212,18 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             } else {
                 constexpr int sl = Staging<Sys, R>::slot(C);
                 static_assert(Staging<Sys, R>::staged(C), "off-centre read of an unstaged component");
-                if constexpr (AX == 2) return q[sl][H + S];
+                if constexpr (AX == 2) return q[sl][DOWN ? H - S : H + S];
                 else if constexpr (AX == 0) return lds[sl][ly][lx + S];
                 else return lds[sl][ly + S][lx];
             }
         }
     };
 
-    // Source value as the stage sees it (the adjoint's first reverse stage
-    // reads b_3 * lambda' without materialising it).
-    __device__ __forceinline__ static double src_at(const StageArgs &a, int c, int i, int j, int k) {
-        const double v = load_cell<H>(a.src, a, c, i, j, k);
+    // Source value of component C at column offset `col` of the plane whose
+    // offset is `zoff` (the adjoint's first reverse stage reads b_3 * lambda'
+    // without materialising it).
+    template <int C> __device__ __forceinline__ static double src_at(const StageArgs &a, int col, int zoff) {
+        const double v = (a.src + C * a.sc + zoff)[col];
         if constexpr (ADJ) return a.src_scale * v;
         else return v;
     }
@@ -244,9 +235,9 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     //   k = F(src);  acc = (first ? init : acc) + b*k;  argn = init + a_next*k
     // Adjoint (reverse stage j of the discrete adjoint RK4, Sys = transpose system):
     //   nu = L^T mu;  acc = (first ? lambda' : acc) + nu;  argn = b_{j-1}*lambda' + a_j*nu
+    template <bool DOWN>
     __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
                                                     double *lds_raw) {
-        Plane *lds = reinterpret_cast<Plane *>(lds_raw);
         const int tid = threadIdx.x;
         const int lane = tid & 63;
         const int row = tid >> 6;
@@ -264,24 +255,32 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             write_argn = stage < kStage3;
             read_init = write_argn && !is_first;
         }
+        const int wrapx = a.ghost_mode == 0 && a.periodic[0];
+        const int wrapy = a.ghost_mode == 0 && a.periodic[1];
+        const int wrapz = a.ghost_mode == 0 && a.periodic[2];
+        auto zoff_of = [&](int k) { return map_axis(k, a.nz, wrapz, a.G) * a.sk; };
 
-        // halo cells owned by this thread (LDS coordinates, -H based)
-        int hlx[HPT], hly[HPT];
+        // column offsets of this lane's own column and of its halo cells
+        const int qcol = map_axis(gi, a.nx, wrapx, a.G) + map_axis(gj, a.ny, wrapy, a.G) * a.sj;
+        int hcol[HPT], hlds[HPT];
         bool hok[HPT];
 #pragma unroll
         for (int m = 0; m < HPT; ++m) {
             const int h = tid + m * NT;
-            hok[m] = h < NHALO;
+            int hx, hy;
             if (h < 2 * H * TX) { // rows above/below the tile
                 const int r = h / TX;
-                hly[m] = r < H ? r - H : TY + (r - H);
-                hlx[m] = h % TX;
+                hy = r < H ? r - H : TY + (r - H);
+                hx = h % TX;
             } else { // columns left/right of the tile
                 const int e = h - 2 * H * TX;
                 const int mm = e % (2 * H);
-                hly[m] = e / (2 * H);
-                hlx[m] = mm < H ? mm - H : TX + (mm - H);
+                hy = e / (2 * H);
+                hx = mm < H ? mm - H : TX + (mm - H);
             }
+            hok[m] = h < NHALO;
+            hcol[m] = map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj;
+            hlds[m] = (hy + H) * LW + hx + H;
         }
 
         // ABL damping: the x/y part of the squared distance to the bulk box is fixed per lane
@@ -295,30 +294,37 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
 
         double q[NS][Q];
         double hv[NS][HPT];
-
-        static_for<0, NC>([&]<int C>() {
-            if constexpr (Staging<Sys, R>::staged(C)) {
-                constexpr int sl = Staging<Sys, R>::slot(C);
-#pragma unroll
-                for (int m = 0; m < Q; ++m) q[sl][m] = src_at(a, C, gi, gj, z0 - H + m);
-#pragma unroll
-                for (int m = 0; m < HPT; ++m) hv[sl][m] = hok[m] ? src_at(a, C, x0 + hlx[m], y0 + hly[m], z0) : 0.0;
-            }
-        });
-
-        for (int z = z0; z < z1; ++z) {
-            // prefetch the next plane of the queue and of the halo
-            double qn[NS], hvn[NS][HPT];
-            const bool more = z + 1 < z1;
+        {
+            const int zh = zoff_of(DOWN ? z1 - 1 : z0);
             static_for<0, NC>([&]<int C>() {
                 if constexpr (Staging<Sys, R>::staged(C)) {
                     constexpr int sl = Staging<Sys, R>::slot(C);
-                    qn[sl] = more ? src_at(a, C, gi, gj, z + 1 + H) : 0.0;
 #pragma unroll
-                    for (int m = 0; m < HPT; ++m)
-                        hvn[sl][m] = (more && hok[m]) ? src_at(a, C, x0 + hlx[m], y0 + hly[m], z + 1) : 0.0;
+                    for (int m = 0; m < Q; ++m) q[sl][m] = src_at<C>(a, qcol, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
+#pragma unroll
+                    for (int m = 0; m < HPT; ++m) hv[sl][m] = hok[m] ? src_at<C>(a, hcol[m], zh) : 0.0;
                 }
             });
+        }
+
+        for (int t = 0; t < z1 - z0; ++t) {
+            const int z = DOWN ? z1 - 1 - t : z0 + t;
+            const int zn = DOWN ? z - 1 : z + 1; // next plane
+            const bool more = t + 1 < z1 - z0;
+            // prefetch the next plane of the queue and of the halo
+            double qn[NS], hvn[NS][HPT];
+            {
+                const int zq = zoff_of(DOWN ? zn - H : zn + H);
+                const int zh = zoff_of(zn);
+                static_for<0, NC>([&]<int C>() {
+                    if constexpr (Staging<Sys, R>::staged(C)) {
+                        constexpr int sl = Staging<Sys, R>::slot(C);
+                        qn[sl] = more ? src_at<C>(a, qcol, zq) : 0.0;
+#pragma unroll
+                        for (int m = 0; m < HPT; ++m) hvn[sl][m] = (more && hok[m]) ? src_at<C>(a, hcol[m], zh) : 0.0;
+                    }
+                });
+            }
 
             // centre values of this cell: source, init and old acc
             const int off = gi + gj * a.sj + z * a.sk;
@@ -331,17 +337,19 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                     if constexpr (ADJ) v = a.src_scale * v;
                     cen[C] = v;
                 }
-                initc[C] = (active && read_init) ? a.init[off + C * a.sc] : cen[C];
-                accc[C] = (active && !is_first) ? a.acc[off + C * a.sc] : 0.0;
+                initc[C] = (active && read_init) ? __builtin_nontemporal_load(&a.init[off + C * a.sc]) : cen[C];
+                accc[C] = (active && !is_first) ? __builtin_nontemporal_load(&a.acc[off + C * a.sc]) : 0.0;
             });
 
-            __syncthreads(); // previous plane's LDS reads are done
+            // this plane into one half of the double-buffered LDS: the other
+            // half may still be read by waves finishing the previous plane
+            double *buf = lds_raw + (t & 1) * LDS_PLANE;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                lds[s][row + H][lane + H] = q[s][H];
+                buf[s * LH * LW + (row + H) * LW + lane + H] = q[s][H];
 #pragma unroll
                 for (int m = 0; m < HPT; ++m)
-                    if (hok[m]) lds[s][hly[m] + H][hlx[m] + H] = hv[s][m];
+                    if (hok[m]) buf[s * LH * LW + hlds[m]] = hv[s][m];
             }
             __syncthreads();
 
@@ -352,7 +360,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                     const int d2 = cxy2 + dzv * dzv;
                     sigma = d2 == 0 ? 0.0 : a.sigma_tab[d2];
                 }
-                const View v{q, lds, cen, a, lane + H, row + H};
+                const View<DOWN> v{q, reinterpret_cast<const Plane *>(buf), cen, a, lane + H, row + H};
                 static_for<0, NC>([&]<int C>() {
                     double k = Sys::template user<N, C>(v, a);
                     if constexpr (R > 0) {
@@ -363,14 +371,17 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                         k = k + a.ko_strength * ko_term;
                     }
                     if (T > 0) k = k - sigma * (cen[C] - a.target[C]);
+                    double accv, argv = 0.0;
                     if constexpr (ADJ) {
-                        a.acc[off + C * a.sc] = (is_first ? initc[C] : accc[C]) + k;
-                        if (write_argn) a.argn[off + C * a.sc] = a.c_init_argn * initc[C] + a.a_next * k;
+                        accv = (is_first ? initc[C] : accc[C]) + k;
+                        if (write_argn) argv = a.c_init_argn * initc[C] + a.a_next * k;
                     } else {
-                        if (is_first) a.acc[off + C * a.sc] = initc[C] + a.b * k;
-                        else a.acc[off + C * a.sc] = accc[C] + a.b * k;
-                        if (write_argn) a.argn[off + C * a.sc] = initc[C] + a.a_next * k;
+                        if (is_first) accv = initc[C] + a.b * k;
+                        else accv = accc[C] + a.b * k;
+                        if (write_argn) argv = initc[C] + a.a_next * k;
                     }
+                    __builtin_nontemporal_store(accv, &a.acc[off + C * a.sc]);
+                    if (write_argn) __builtin_nontemporal_store(argv, &a.argn[off + C * a.sc]);
                 });
             }
 
@@ -401,7 +412,11 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
         const int tiy = t2 % a.tiles_y;
         const int zc = t2 / a.tiles_y;
         const int z0 = zc * a.zchunk;
-        run_tile(a, tix * TX, tiy * TY, z0, min(z0 + a.zchunk, a.nz), lds);
+        const int z1 = min(z0 + a.zchunk, a.nz);
+        // alternate the march direction per chunk so that the planes two
+        // neighbouring chunks share are read by both at about the same time
+        if (a.zalt && (zc & 1)) run_tile<true>(a, tix * TX, tiy * TY, z0, z1, lds);
+        else run_tile<false>(a, tix * TX, tiy * TY, z0, z1, lds);
     }
 };
 
@@ -423,6 +438,7 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), 0, stream, a); break;
     case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), 0, stream, a); break;
     case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ>), grid, dim3(512), 0, stream, a); break;
+    case 16: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 16, ADJ>), grid, dim3(1024), 0, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
