@@ -73,6 +73,65 @@ def cpu_baseline(case, state, budget_s: float):
     }
 
 
+def ensemble_leg(count: int, steps: int, tune: bool):
+    """Batched vs plain on the ensemble workload: `count` periodic (phi, Pi) instances (side in
+    {32,48,64,96,128}, n in {2,3,4}, KO order n), `steps` RK4 steps each.
+      batched: one Batch, graph-replayed steps, one persistent launch per stage per stencil group;
+      plain:   the instances one after another, each with the single-box solver and eager per-step
+               launches, summed device time."""
+    import torch
+
+    from treesap_amd import Batch, Solver
+    from treesap_amd.workloads import ensemble_params, gaussian_state_device
+
+    params = ensemble_params(count=count)
+    cells = sum(c.cells for c, _ in params)
+    b = Batch([c for c, _ in params])
+    for i, (case, centre) in enumerate(params):
+        b.set_state(i, gaussian_state_device(case, centre))
+    torch.cuda.synchronize()
+    cfgs = [b.kernel]
+    if tune:
+        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=1) for ty in (8, 16) for zc in (0, 64)
+                for gb in (0, 512)]
+    best = None
+    for cfg in cfgs:
+        b.kernel = cfg
+        b.graph_rk4_steps(2)
+        ms = b.time_rk4(4, use_graph=True) / 4
+        if best is None or ms < best[0]:
+            best = (ms, dict(cfg))
+    b.kernel = best[1]
+    for i, (case, centre) in enumerate(params):
+        b.set_state(i, gaussian_state_device(case, centre))
+    b.graph_rk4_steps(2)
+    batched_ms = b.time_rk4(steps, use_graph=True)
+    b.close()
+    torch.cuda.synchronize()
+
+    plain_ms = 0.0
+    for case, centre in params:
+        s = Solver(case)
+        s.set_state(gaussian_state_device(case, centre))
+        plain_ms += s.time_rk4(steps, use_graph=False)
+        s.close()
+    batched = cells * steps / (batched_ms / 1e3)
+    plain = cells * steps / (plain_ms / 1e3)
+    return {
+        "workload": f"ensemble: {count} periodic (phi,Pi) instances, sides {{32,48,64,96,128}}, n in {{2,3,4}}, "
+                    f"KO order n, sigma U[0.05,0.3], {steps} RK4 steps each",
+        "instances": count,
+        "cells": cells,
+        "steps": steps,
+        "batched_cell_updates_per_s": batched,
+        "plain_cell_updates_per_s": plain,
+        "batched_vs_plain": batched / plain,
+        "batched_s": batched_ms / 1e3,
+        "plain_s": plain_ms / 1e3,
+        "batch_kernel": best[1],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +143,9 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--kernel", default=None,
                     help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt] (skips the autotuner)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-vs-plain ensemble leg")
+    ap.add_argument("--ensemble-count", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -151,6 +213,15 @@ def main() -> int:
     log(f"[bench] {args.steps} steps: wall {wall * 1e3:.1f} ms, device {dev_ms:.1f} ms, "
         f"{value:.3e} cell-updates/s, per-stage GB/s {[round(x) for x in stage_gbs]}")
 
+    ens = None
+    if not args.no_ensemble:
+        log(f"[bench] ensemble leg: {args.ensemble_count} instances x {args.ensemble_steps} steps ...")
+        t0 = time.perf_counter()
+        ens = ensemble_leg(args.ensemble_count, args.ensemble_steps, tune=not args.no_tune)
+        log(f"[bench] ensemble: batched {ens['batched_cell_updates_per_s']:.3e} cu/s, plain "
+            f"{ens['plain_cell_updates_per_s']:.3e} cu/s, ratio {ens['batched_vs_plain']:.2f} "
+            f"({time.perf_counter() - t0:.1f} s)")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline ...")
@@ -189,6 +260,7 @@ def main() -> int:
             "stage_gbs_eager": stage_gbs,
         },
         "cpu_baseline": cpu,
+        "batched": ens,
     }
     print(json.dumps(line), flush=True)
     return 0

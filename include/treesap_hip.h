@@ -173,6 +173,41 @@ tsp_status tsp_sync(tsp_solver *s);
  *   ptr[i + j*sj + k*sk + c*sc]   with ptr pointing at (0,0,0,0). */
 tsp_status tsp_register_view(tsp_solver *s, int32_t role, double **ptr, int64_t *sj, int64_t *sk, int64_t *sc);
 
+/* Batched mode (no reference counterpart): many independent (phi, Pi)
+ * models of uneven sizes, stencil widths and KO orders advanced in lockstep.
+ * Per RK4 stage there is one persistent launch per (stencil width, KO order)
+ * group present in the batch, covering every instance of the group; steps can
+ * be replayed from a captured hipGraph. Each instance's result is
+ * bit-identical to solving it alone with tsp_create/tsp_rk4_steps (the same
+ * tile body runs). */
+typedef struct tsp_batch tsp_batch;
+
+typedef struct tsp_batch_cfg {
+    int32_t tile_y;      /* 8 or 16 rows per workgroup (tile_x 64) */
+    int32_t zchunk;      /* z planes per work item; 0 = min(nz, 32) per instance */
+    int32_t grid_blocks; /* persistent grid size; 0 = one workgroup per work item */
+    int32_t zalt;        /* 1: odd z chunks march downwards */
+} tsp_batch_cfg;
+
+/* Every descriptor must pass tsp_create's validation, use TSP_SYS_WAVE_PHIPI
+ * and a (stencil_n, ko_order) pair in {(1,0),(1,1),(2,0),(2,2),(2,3),(3,0),
+ * (3,3),(4,0),(4,4)}. All registers live in one pool allocation. */
+tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out);
+tsp_status tsp_batch_destroy(tsp_batch *b);
+tsp_status tsp_batch_size(tsp_batch *b, int32_t *count, int64_t *cells);
+/* Per-instance initial condition / state, same layout as tsp_set_state. */
+tsp_status tsp_batch_set_state(tsp_batch *b, int32_t index, const double *src, int32_t on_device);
+tsp_status tsp_batch_get_state(tsp_batch *b, int32_t index, double *dst, int32_t on_device);
+tsp_status tsp_batch_get_time(tsp_batch *b, int32_t index, double *time, int64_t *steps);
+tsp_status tsp_batch_get_kernel(tsp_batch *b, tsp_batch_cfg *cfg);
+tsp_status tsp_batch_set_kernel(tsp_batch *b, const tsp_batch_cfg *cfg);
+/* `nsteps` RK4 steps of every instance: eager launches, or graph replay (two
+ * steps per graph, odd remainder eager). */
+tsp_status tsp_batch_rk4_steps(tsp_batch *b, int64_t nsteps);
+tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
+tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms);
+tsp_status tsp_batch_sync(tsp_batch *b);
+
 /* Algorithmic HBM bytes of one RK4 step per valid cell (128 * ncomp). */
 int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d);
 

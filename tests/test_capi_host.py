@@ -111,3 +111,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setenv("TSP_LIBRARY", str(tmp_path / "absent.so"))
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_batch_validation():
+    from treesap_amd import Batch
+
+    good = Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01)
+    with pytest.raises(_lib.TspError, match="batched mode runs the"):
+        Batch([good, good.replace(system="phiu")])
+    with pytest.raises(_lib.TspError, match="no batched kernel"):
+        Batch([good.replace(stencil_n=3, ghost=4, ko_order=4)])
+    with pytest.raises(_lib.TspError, match="instance 1: .*stencil_n"):
+        Batch([good, good.replace(stencil_n=0)])
+    L = _lib.lib()
+    assert L.tsp_batch_create(None, 0, None, None) == _lib.INVALID_ARG
+    assert L.tsp_batch_rk4_steps(None, 1) == _lib.INVALID_ARG
+    assert L.tsp_batch_destroy(None) == _lib.OK

@@ -1,0 +1,96 @@
+"""Batched mode: every instance of a batch must equal (bitwise) its solve on its
+own, and the oracle, for every batch configuration and for graph replay."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from treesap_amd import Batch, Solver
+from treesap_amd.workloads import ensemble, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_cases():
+    specs = [
+        ((70, 21, 19), 4, 4, (0, 0, 0), 5),
+        ((24, 20, 16), 2, 3, (1, 1, 1), 0),
+        ((33, 18, 22), 3, 3, (1, 0, 1), 0),
+        ((17, 9, 12), 1, 1, (0, 1, 0), 3),
+        ((40, 24, 20), 2, 2, (1, 1, 1), 0),
+        ((32, 32, 32), 4, 0, (1, 1, 1), 0),
+        ((48, 16, 8), 2, 0, (0, 0, 0), 2),
+        ((64, 20, 30), 3, 0, (1, 1, 1), 0),
+    ]
+    out = []
+    for k, (n, sn, ko, per, abl) in enumerate(specs):
+        case, state = small_case("phipi", n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=11 + k)
+        case = case.replace(dt=case.dt * (1.0 + 0.1 * k))  # every instance has its own time step
+        out.append((case, state))
+    return out
+
+
+BATCH_CFGS = [
+    {"tile_y": 16, "zchunk": 0, "grid_blocks": 0, "zalt": 1},
+    {"tile_y": 8, "zchunk": 5, "grid_blocks": 7, "zalt": 0},
+    {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1},
+]
+
+
+@pytest.mark.parametrize("cfg", BATCH_CFGS)
+def test_batch_equals_single_solves_and_oracle(cfg):
+    _say(f"[batch] cfg {cfg}")
+    cases = mixed_cases()
+    b = Batch([c for c, _ in cases])
+    b.kernel = cfg
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.rk4_steps(5)
+    for i, (case, st) in enumerate(cases):
+        got = b.get_state(i)
+        s = Solver(case)
+        s.set_state(st)
+        s.rk4_steps(5)
+        assert np.array_equal(got, s.get_state()), i
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(5)
+        assert np.array_equal(got, o.get_state()), i
+        assert b.time(i) == o.time
+    assert b.steps == 5
+
+
+def test_batch_graph_equals_eager():
+    _say("[batch] graph")
+    cases = mixed_cases()
+    a = Batch([c for c, _ in cases])
+    g = Batch([c for c, _ in cases])
+    for i, (_, st) in enumerate(cases):
+        a.set_state(i, st)
+        g.set_state(i, st)
+    a.rk4_steps(7)
+    g.graph_rk4_steps(3)
+    g.graph_rk4_steps(4)
+    for i in range(len(cases)):
+        assert np.array_equal(a.get_state(i), g.get_state(i))
+
+
+def test_small_ensemble_workload():
+    """A 12-instance slice of the ensemble workload (sides 32..64, n in {2,3,4}) for 20 steps."""
+    _say("[batch] ensemble slice")
+    inst, _ = ensemble(count=12, sides=(32, 48, 64))
+    b = Batch([c for c, _ in inst])
+    for i, (_, st) in enumerate(inst):
+        b.set_state(i, st)
+    b.graph_rk4_steps(20)
+    for i, (case, st) in enumerate(inst[:4]):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(20)
+        assert np.array_equal(b.get_state(i), o.get_state()), i

@@ -65,6 +65,18 @@ class TspKernelCfg(C.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class TspBatchCfg(C.Structure):
+    _fields_ = [
+        ("tile_y", C.c_int32),
+        ("zchunk", C.c_int32),
+        ("grid_blocks", C.c_int32),
+        ("zalt", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class TspError(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
         super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {message}")
@@ -98,6 +110,18 @@ PROTOTYPES = {
     "tsp_sync": (C.c_int, [_VP]),
     "tsp_register_view": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP), _I64P, _I64P, _I64P]),
     "tsp_rk4_bytes_per_cell": (C.c_int64, [C.POINTER(TspDesc)]),
+    "tsp_batch_create": (C.c_int, [C.POINTER(TspDesc), C.c_int32, _VP, C.POINTER(_VP)]),
+    "tsp_batch_destroy": (C.c_int, [_VP]),
+    "tsp_batch_size": (C.c_int, [_VP, C.POINTER(C.c_int32), _I64P]),
+    "tsp_batch_set_state": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32]),
+    "tsp_batch_get_state": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32]),
+    "tsp_batch_get_time": (C.c_int, [_VP, C.c_int32, _DP, _I64P]),
+    "tsp_batch_get_kernel": (C.c_int, [_VP, C.POINTER(TspBatchCfg)]),
+    "tsp_batch_set_kernel": (C.c_int, [_VP, C.POINTER(TspBatchCfg)]),
+    "tsp_batch_rk4_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_batch_graph_rk4_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_batch_time_rk4": (C.c_int, [_VP, C.c_int64, C.c_int32, _DP]),
+    "tsp_batch_sync": (C.c_int, [_VP]),
 }
 
 _lib = None

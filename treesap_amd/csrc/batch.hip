@@ -1,0 +1,445 @@
+// batch.hip -- the batched half of the C ABI (tsp_batch_*) and the batched
+// stage kernels of the (phi, Pi) system.
+//
+// A batch is many independent single-box models advanced together. All
+// registers of all instances live in one pool allocation (each register keeps
+// the single-box layout, see capi.hip); a device instance table and a work
+// list of (instance, tile, z chunk) items drive one persistent launch per RK4
+// stage. The reference has no batched mode: this is the data-parallel way to
+// run an ensemble of SingleLevelRK4Integrator solves
+// (integrators/SingleLevelRK4Integrator.hpp:95-157) of uneven sizes.
+#include "batch_kernel.hpp"
+#include "capi_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+namespace tsp {
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, hipStream_t stream) {
+    hipError_t e = hipErrorInvalidValue;
+    static_for<0, kBatchVariants>([&]<int V>() {
+        if (variant == V) e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, stream);
+    });
+    return e;
+}
+} // namespace tsp
+
+using namespace tsp_host;
+
+struct tsp_batch {
+    int count = 0;
+    std::vector<tsp_desc> d;
+    std::vector<Layout> L;
+    std::vector<int64_t> reg_off; // 4 per instance: element offset of each register's base in the pool
+    double *pool = nullptr;
+    int64_t pool_elems = 0;
+    std::vector<double *> sigma;  // per instance (null without ABL)
+    std::vector<tsp::BatchInstance> h_inst;
+    tsp::BatchInstance *d_inst = nullptr;
+    tsp::BatchWork *d_work = nullptr;
+    int nwork = 0;
+    int group_off[tsp::kBatchVariants] = {}; // work items of variant v: [group_off[v], group_off[v] + group_n[v])
+    int group_n[tsp::kBatchVariants] = {};
+    int role[4] = {0, 1, 2, 3};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    tsp_batch_cfg cfg{16, 0, 0, 1};
+    tsp_batch_cfg work_cfg{-1, -1, -1, -1}; // configuration the work list was built for
+    std::vector<double> time;
+    int64_t steps = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    tsp_batch_cfg graph_cfg{-1, -1, -1, -1};
+};
+
+namespace {
+
+double *batch_origin(const tsp_batch *b, int i, int reg) {
+    const Layout &L = b->L[i];
+    return b->pool + b->reg_off[4 * i + reg] + kXOff + (int64_t)L.G * L.sj + (int64_t)L.G * L.sk;
+}
+
+tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
+    if (!c) return fail(TSP_INVALID_ARG, "batch config is NULL");
+    if (c->tile_y != 8 && c->tile_y != 16) return fail(TSP_INVALID_ARG, "batch tile_y must be 8 or 16");
+    if (c->zchunk < 0) return fail(TSP_INVALID_ARG, "zchunk must be >= 0 (0: automatic)");
+    if (c->grid_blocks < 0) return fail(TSP_INVALID_ARG, "grid_blocks must be >= 0 (0: one block per work item)");
+    if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
+    return TSP_OK;
+}
+
+int auto_zchunk(int nz) { return std::min(nz, 32); }
+
+// (Re)build the device work list: instances in decreasing cell count, each
+// split into 64 x tile_y column tiles and z chunks.
+tsp_status build_work(tsp_batch *b) {
+    if (b->d_work && std::memcmp(&b->work_cfg, &b->cfg, sizeof(tsp_batch_cfg)) == 0) return TSP_OK;
+    std::vector<int> order(b->count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int64_t cx = (int64_t)b->d[x].n[0] * b->d[x].n[1] * b->d[x].n[2];
+        const int64_t cy = (int64_t)b->d[y].n[0] * b->d[y].n[1] * b->d[y].n[2];
+        return cx > cy;
+    });
+    std::vector<tsp::BatchWork> work;
+    const int ty = b->cfg.tile_y;
+    for (int v = 0; v < tsp::kBatchVariants; ++v) {
+        b->group_off[v] = (int)work.size();
+        for (int i : order) {
+            if (b->h_inst[i].variant != v) continue;
+            const tsp_desc &d = b->d[i];
+            const int zc = b->cfg.zchunk > 0 ? std::min(b->cfg.zchunk, d.n[2]) : auto_zchunk(d.n[2]);
+            b->h_inst[i].zchunk = zc;
+            const int nzc = (d.n[2] + zc - 1) / zc;
+            for (int c = 0; c < nzc; ++c)
+                for (int y0 = 0; y0 < d.n[1]; y0 += ty)
+                    for (int x0 = 0; x0 < d.n[0]; x0 += 64) work.push_back(tsp::BatchWork{i, x0, y0, c});
+        }
+        b->group_n[v] = (int)work.size() - b->group_off[v];
+    }
+    if (work.size() > 0x7fffffff) return fail(TSP_UNSUPPORTED, "work list too long");
+    if (b->d_work) {
+        TSP_HIP(hipStreamSynchronize(b->stream));
+        (void)hipFree(b->d_work);
+        b->d_work = nullptr;
+    }
+    TSP_HIP(hipMalloc((void **)&b->d_work, sizeof(tsp::BatchWork) * work.size()));
+    TSP_HIP(hipMemcpy(b->d_work, work.data(), sizeof(tsp::BatchWork) * work.size(), hipMemcpyHostToDevice));
+    TSP_HIP(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(tsp::BatchInstance) * b->count, hipMemcpyHostToDevice));
+    b->nwork = (int)work.size();
+    b->work_cfg = b->cfg;
+    return TSP_OK;
+}
+
+void drop_batch_graphs(tsp_batch *b) {
+    for (auto &g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// One RK4 step of every instance: per stage, one persistent launch per
+// (stencil width, KO order) group present in the batch.
+tsp_status enqueue_batch_step(tsp_batch *b) {
+    int &OLD = b->role[TSP_ROLE_OLD], &NEW = b->role[TSP_ROLE_NEW], &ARG = b->role[TSP_ROLE_ARG],
+        &ARGN = b->role[TSP_ROLE_ARG_NEXT];
+    for (int stage = 0; stage < 4; ++stage) {
+        if (stage > 0) std::swap(ARG, ARGN);
+        for (int v = 0; v < tsp::kBatchVariants; ++v) {
+            if (b->group_n[v] == 0) continue;
+            tsp::BatchArgs a{};
+            a.inst = b->d_inst;
+            a.work = b->d_work + b->group_off[v];
+            a.nwork = b->group_n[v];
+            a.stage = stage;
+            a.src = stage == 0 ? OLD : ARG;
+            a.init = OLD;
+            a.acc = NEW;
+            a.argn = stage < 3 ? ARGN : -1;
+            a.zalt = b->cfg.zalt;
+            a.ghost_mode = 0;
+            const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
+            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->stream));
+        }
+    }
+    std::swap(OLD, NEW);
+    return TSP_OK;
+}
+
+void batch_advance_clock(tsp_batch *b) {
+    b->steps++;
+    for (int i = 0; i < b->count; ++i) b->time[i] += b->d[i].dt;
+}
+
+} // namespace
+
+extern "C" {
+
+tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out) {
+    if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!descs || count < 1) return fail(TSP_INVALID_ARG, "need at least one instance");
+    for (int i = 0; i < count; ++i) {
+        tsp_status st = validate(&descs[i]);
+        if (st != TSP_OK) return fail(st, "instance " + std::to_string(i) + ": " + g_err);
+        if (descs[i].system != TSP_SYS_WAVE_PHIPI)
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": batched mode runs the (phi,Pi) system");
+        if (tsp::batch_variant(descs[i].stencil_n, descs[i].ko_order) < 0)
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": no batched kernel for stencil_n=" +
+                                             std::to_string(descs[i].stencil_n) +
+                                             " ko_order=" + std::to_string(descs[i].ko_order));
+    }
+    tsp_batch *b = new (std::nothrow) tsp_batch();
+    if (!b) return fail(TSP_OOM, "host allocation failed");
+    auto cleanup = [&](tsp_status e) {
+        tsp_batch_destroy(b);
+        return e;
+    };
+    b->count = count;
+    b->d.assign(descs, descs + count);
+    b->time.assign(count, 0.0);
+    b->sigma.assign(count, nullptr);
+    int64_t off = 0;
+    for (int i = 0; i < count; ++i) {
+        b->L.push_back(layout_of(descs[i]));
+        for (int r = 0; r < 4; ++r) {
+            b->reg_off.push_back(off);
+            off += (b->L[i].elems + 31) / 32 * 32; // 256-byte aligned registers
+        }
+    }
+    b->pool_elems = off;
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreateWithFlags"));
+        b->own_stream = true;
+    }
+    hipError_t e = hipMalloc((void **)&b->pool, sizeof(double) * b->pool_elems);
+    if (e == hipErrorOutOfMemory) return cleanup(fail(TSP_OOM, "hipMalloc of the register pool failed (out of memory)"));
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(pool)"));
+    e = hipMemsetAsync(b->pool, 0, sizeof(double) * b->pool_elems, b->stream);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemsetAsync(pool)"));
+    b->h_inst.resize(count);
+    for (int i = 0; i < count; ++i) {
+        const tsp_desc &d = descs[i];
+        const Layout &L = b->L[i];
+        if (d.abl_thickness > 0) {
+            std::vector<double> tab = abl_sigma_table(d);
+            e = hipMalloc((void **)&b->sigma[i], sizeof(double) * tab.size());
+            if (e == hipSuccess)
+                e = hipMemcpy(b->sigma[i], tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return cleanup(hip_fail(e, "sigma table"));
+        }
+        tsp::BatchInstance &I = b->h_inst[i];
+        std::memset(&I, 0, sizeof(I));
+        for (int r = 0; r < 4; ++r) I.reg[r] = batch_origin(b, i, r);
+        I.sigma_tab = b->sigma[i];
+        I.nx = d.n[0];
+        I.ny = d.n[1];
+        I.nz = d.n[2];
+        I.G = L.G;
+        I.sj = (int32_t)L.sj;
+        I.sk = (int32_t)L.sk;
+        I.sc = (int32_t)L.sc;
+        I.variant = tsp::batch_variant(d.stencil_n, d.ko_order);
+        for (int a = 0; a < 3; ++a) I.periodic[a] = d.periodic[a];
+        I.dirichlet = d.bc == TSP_BC_DIRICHLET;
+        I.abl_T = d.abl_thickness;
+        I.dt = d.dt;
+        for (int a = 0; a < 3; ++a) {
+            const double dx = (d.prob_hi[a] - d.prob_lo[a]) / (double)d.n[a];
+            I.gf1[a] = 1.0 / dx;
+            I.gf2[a] = 1.0 / (dx * dx);
+        }
+        I.ko_strength = d.ko_strength;
+        I.gamma = d.gamma;
+        for (int c = 0; c < TSP_MAX_COMP; ++c) I.target[c] = d.target[c];
+        // physical ghosts hold their boundary values for good (see tsp_create)
+        for (int r = 0; r < 4; ++r) {
+            tsp::GhostArgs g{};
+            g.p = I.reg[r];
+            g.nx = d.n[0];
+            g.ny = d.n[1];
+            g.nz = d.n[2];
+            g.G = L.G;
+            g.nc = system_ncomp(d.system);
+            g.sj = I.sj;
+            g.sk = I.sk;
+            g.sc = I.sc;
+            for (int a = 0; a < 3; ++a) g.periodic[a] = d.periodic[a];
+            g.dirichlet = I.dirichlet;
+            for (int c = 0; c < TSP_MAX_COMP; ++c) g.target[c] = d.target[c];
+            e = tsp::launch_ghost_fill(g, b->stream);
+            if (e != hipSuccess) return cleanup(hip_fail(e, "ghost prefill"));
+        }
+    }
+    e = hipMalloc((void **)&b->d_inst, sizeof(tsp::BatchInstance) * count);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(instances)"));
+    tsp_status st = build_work(b);
+    if (st != TSP_OK) return cleanup(st);
+    e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamSynchronize"));
+    *out = b;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_destroy(tsp_batch *b) {
+    if (!b) return TSP_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_batch_graphs(b);
+    if (b->pool) (void)hipFree(b->pool);
+    for (double *p : b->sigma)
+        if (p) (void)hipFree(p);
+    if (b->d_inst) (void)hipFree(b->d_inst);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_size(tsp_batch *b, int32_t *count, int64_t *cells) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (count) *count = b->count;
+    if (cells) {
+        int64_t c = 0;
+        for (const tsp_desc &d : b->d) c += (int64_t)d.n[0] * d.n[1] * d.n[2];
+        *cells = c;
+    }
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_set_state(tsp_batch *b, int32_t index, const double *src, int32_t on_device) {
+    if (!b || !src) return fail(TSP_INVALID_ARG, "batch or source is NULL");
+    if (index < 0 || index >= b->count) return fail(TSP_INVALID_ARG, "instance index out of range");
+    const tsp_desc &d = b->d[index];
+    const Layout &L = b->L[index];
+    const int nc = system_ncomp(d.system);
+    const size_t bytes = sizeof(double) * (size_t)d.n[0] * d.n[1] * d.n[2] * nc;
+    const double *dsrc = src;
+    double *staging = nullptr;
+    if (!on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, bytes));
+        hipError_t e = hipMemcpyAsync(staging, src, bytes, hipMemcpyHostToDevice, b->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(staging);
+            return hip_fail(e, "hipMemcpyAsync(state)");
+        }
+        dsrc = staging;
+    }
+    hipError_t e = tsp::launch_unpack(batch_origin(b, index, b->role[TSP_ROLE_OLD]), dsrc, d.n[0], d.n[1], d.n[2], nc,
+                                      (int)L.sj, (int)L.sk, (int)L.sc, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) return hip_fail(e, "batch set_state");
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_get_state(tsp_batch *b, int32_t index, double *dst, int32_t on_device) {
+    if (!b || !dst) return fail(TSP_INVALID_ARG, "batch or destination is NULL");
+    if (index < 0 || index >= b->count) return fail(TSP_INVALID_ARG, "instance index out of range");
+    const tsp_desc &d = b->d[index];
+    const Layout &L = b->L[index];
+    const int nc = system_ncomp(d.system);
+    const size_t bytes = sizeof(double) * (size_t)d.n[0] * d.n[1] * d.n[2] * nc;
+    double *ddst = dst;
+    double *staging = nullptr;
+    if (!on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, bytes));
+        ddst = staging;
+    }
+    hipError_t e = tsp::launch_pack(ddst, batch_origin(b, index, b->role[TSP_ROLE_OLD]), d.n[0], d.n[1], d.n[2], nc,
+                                    (int)L.sj, (int)L.sk, (int)L.sc, b->stream);
+    if (e == hipSuccess && staging) e = hipMemcpyAsync(dst, staging, bytes, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) return hip_fail(e, "batch get_state");
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_get_time(tsp_batch *b, int32_t index, double *time, int64_t *steps) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (index < 0 || index >= b->count) return fail(TSP_INVALID_ARG, "instance index out of range");
+    if (time) *time = b->time[index];
+    if (steps) *steps = b->steps;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_get_kernel(tsp_batch *b, tsp_batch_cfg *cfg) {
+    if (!b || !cfg) return fail(TSP_INVALID_ARG, "batch or cfg is NULL");
+    *cfg = b->cfg;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_set_kernel(tsp_batch *b, const tsp_batch_cfg *cfg) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    tsp_status st = validate_batch_cfg(cfg);
+    if (st != TSP_OK) return st;
+    b->cfg = *cfg;
+    return build_work(b);
+}
+
+tsp_status tsp_batch_rk4_steps(tsp_batch *b, int64_t nsteps) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    tsp_status st = build_work(b);
+    for (int64_t n = 0; n < nsteps && st == TSP_OK; ++n) {
+        st = enqueue_batch_step(b);
+        if (st == TSP_OK) batch_advance_clock(b);
+    }
+    return st;
+}
+
+tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    tsp_status st = build_work(b);
+    if (st != TSP_OK) return st;
+    if (std::memcmp(&b->graph_cfg, &b->cfg, sizeof(tsp_batch_cfg)) != 0) {
+        drop_batch_graphs(b);
+        b->graph_cfg = b->cfg;
+    }
+    const int parity = (int)(b->steps & 1);
+    if (!b->graph[parity] && nsteps >= 2) {
+        int saved[4];
+        std::memcpy(saved, b->role, sizeof(saved));
+        TSP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        st = enqueue_batch_step(b);
+        if (st == TSP_OK) st = enqueue_batch_step(b);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        std::memcpy(b->role, saved, sizeof(saved));
+        if (st != TSP_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&b->graph[parity], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            b->graph[parity] = nullptr;
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+    }
+    int64_t n = 0;
+    for (; n + 2 <= nsteps; n += 2) {
+        TSP_HIP(hipGraphLaunch(b->graph[parity], b->stream));
+        batch_advance_clock(b);
+        batch_advance_clock(b);
+    }
+    if (n < nsteps) return tsp_batch_rk4_steps(b, nsteps - n);
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms) {
+    if (!b || !ms) return fail(TSP_INVALID_ARG, "batch or ms is NULL");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TSP_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    tsp_status st = build_work(b);
+    if (st == TSP_OK && (e = hipEventRecord(e0, b->stream)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
+    if (st == TSP_OK) st = use_graph ? tsp_batch_graph_rk4_steps(b, nsteps) : tsp_batch_rk4_steps(b, nsteps);
+    if (st == TSP_OK && (e = hipEventRecord(e1, b->stream)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
+    if (st == TSP_OK && (e = hipEventSynchronize(e1)) != hipSuccess) st = hip_fail(e, "hipEventSynchronize");
+    float f = 0.f;
+    if (st == TSP_OK && (e = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) st = hip_fail(e, "hipEventElapsedTime");
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+tsp_status tsp_batch_sync(tsp_batch *b) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    TSP_HIP(hipStreamSynchronize(b->stream));
+    return TSP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,144 @@
+// batch_kernel.hpp -- one launch per RK4 stage for many independent boxes.
+//
+// Every instance is a separate (phi, Pi) model with its own box size, stencil
+// width, KO order and strength, boundary handling and time step; all step in
+// lockstep, so the register roles (state_old, state_new, arg, arg_next) rotate
+// identically for all of them and travel as kernel arguments. The work list
+// holds (instance, tile, z chunk) items, largest instances first; a grid of
+// `gridDim.x` persistent blocks strides over it. Each item runs exactly the
+// tile body of the single-box kernel (FusedStage::run_tile), so a batched
+// solve is bit-identical to solving every instance on its own.
+#pragma once
+
+#include "stage_kernel.hpp"
+
+namespace tsp {
+
+struct BatchInstance {
+    double *reg[4];          // register origins (element (0,0,0,0))
+    const double *sigma_tab; // ABL rate table or null
+    int32_t nx, ny, nz, G, sj, sk, sc;
+    int32_t variant;         // index into the batched (stencil, KO order) list
+    int32_t periodic[3];
+    int32_t dirichlet, abl_T, zchunk;
+    double dt;
+    double gf1[3], gf2[3];
+    double ko_strength, gamma;
+    double target[TSP_MAX_COMP];
+};
+
+struct BatchWork {
+    int32_t inst, x0, y0, zc;
+};
+
+struct BatchArgs {
+    const BatchInstance *inst;
+    const BatchWork *work;
+    int32_t nwork;
+    int32_t stage;           // StageKind
+    int32_t src, init, acc, argn; // register indices of the roles for this stage
+    int32_t zalt;
+    int32_t ghost_mode;
+};
+
+// (stencil width, KO order) pairs the batched kernel carries
+constexpr int kBatchVariants = 9;
+constexpr int kBatchN[kBatchVariants] = {1, 1, 2, 2, 2, 3, 3, 4, 4};
+constexpr int kBatchR[kBatchVariants] = {0, 1, 0, 2, 3, 0, 3, 0, 4};
+
+inline int batch_variant(int n, int r) {
+    for (int v = 0; v < kBatchVariants; ++v)
+        if (kBatchN[v] == n && kBatchR[v] == r) return v;
+    return -1;
+}
+
+// One (stencil width, KO order) group: a persistent grid strides over the
+// group's work items.
+template <class Sys, int N, int R, int TY> struct BatchedStage {
+    using FS = FusedStage<Sys, N, R, TY>;
+
+    __device__ __forceinline__ static void run(const BatchInstance *__restrict__ inst,
+                                               const BatchWork *__restrict__ work, const BatchArgs &b, double *lds) {
+        for (int w = blockIdx.x; w < b.nwork; w += gridDim.x) {
+            const BatchWork wk = work[w];
+            const BatchInstance &I = inst[wk.inst];
+            StageArgs a;
+            a.src = I.reg[b.src];
+            a.init = I.reg[b.init];
+            a.acc = I.reg[b.acc];
+            a.argn = b.argn >= 0 ? I.reg[b.argn] : nullptr;
+            a.sigma_tab = I.sigma_tab;
+            a.nx = I.nx;
+            a.ny = I.ny;
+            a.nz = I.nz;
+            a.G = I.G;
+            a.sj = I.sj;
+            a.sk = I.sk;
+            a.sc = I.sc;
+            a.stage = b.stage;
+            a.ghost_mode = b.ghost_mode;
+            a.periodic[0] = I.periodic[0];
+            a.periodic[1] = I.periodic[1];
+            a.periodic[2] = I.periodic[2];
+            a.dirichlet = I.dirichlet;
+            a.abl_T = I.abl_T;
+            a.zchunk = I.zchunk;
+            a.tiles_x = a.tiles_y = a.nzc = 0;
+            a.xcd_remap = 0;
+            a.zalt = b.zalt;
+            // integrators/SingleLevelRK4Integrator.hpp:71-74,119-121 (the same
+            // products the host forms for a single box)
+            const int s = b.stage;
+            a.b = s == kEuler ? I.dt : I.dt * ((s == 0 || s == 3) ? 1.0 / 6.0 : 1.0 / 3.0);
+            a.a_next = I.dt * (s == 2 ? 1.0 : 0.5);
+            a.c_init_argn = 0.0;
+            a.src_scale = 1.0;
+            a.gf1[0] = I.gf1[0];
+            a.gf1[1] = I.gf1[1];
+            a.gf1[2] = I.gf1[2];
+            a.gf2[0] = I.gf2[0];
+            a.gf2[1] = I.gf2[1];
+            a.gf2[2] = I.gf2[2];
+            a.ko_strength = I.ko_strength;
+            a.gamma = I.gamma;
+#pragma unroll
+            for (int c = 0; c < TSP_MAX_COMP; ++c) a.target[c] = I.target[c];
+            const int z0 = wk.zc * I.zchunk;
+            const int z1 = min(z0 + I.zchunk, I.nz);
+            __syncthreads(); // the previous item's waves may still read LDS
+            if (b.zalt && (wk.zc & 1)) FS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds);
+            else FS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds);
+        }
+    }
+};
+
+// The instance table and work list are separate restrict-qualified arguments:
+// nothing the kernel stores can alias them, so their uniform loads can be
+// scalar loads.
+template <class Sys, int N, int R, int TY>
+__global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInstance *__restrict__ inst,
+                                                                const BatchWork *__restrict__ work,
+                                                                const BatchArgs b) {
+    __shared__ double lds[FusedStage<Sys, N, R, TY>::LDS_DOUBLES];
+    BatchedStage<Sys, N, R, TY>::run(inst, work, b, lds);
+}
+
+template <class Sys, int N, int R> hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid,
+                                                            hipStream_t stream) {
+    switch (tile_y) {
+    case 8:
+        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 8>), dim3(grid), dim3(512), 0, stream, b.inst, b.work, b);
+        break;
+    case 16:
+        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 16>), dim3(grid), dim3(1024), 0, stream, b.inst, b.work,
+                           b);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// launch one variant group of the (phi, Pi) batch
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, hipStream_t stream);
+
+} // namespace tsp

@@ -154,3 +154,99 @@ class Solver:
 
     def rk4_bytes_per_cell(self) -> int:
         return int(_lib.lib().tsp_rk4_bytes_per_cell(C.byref(self._desc)))
+
+
+class Batch:
+    """Many independent (phi, Pi) models advanced in lockstep by one launch per RK4 stage."""
+
+    def __init__(self, cases, stream=None):
+        self.cases = list(cases)
+        arr = (_lib.TspDesc * len(self.cases))(*[make_desc(c) for c in self.cases])
+        self._descs = arr
+        handle = C.c_void_p()
+        st = _lib.lib().tsp_batch_create(arr, len(self.cases), C.c_void_p(stream), C.byref(handle))
+        _lib.check(st, "tsp_batch_create")
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().tsp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cases)
+
+    @property
+    def cells(self) -> int:
+        n = C.c_int32()
+        c = C.c_int64()
+        _lib.check(_lib.lib().tsp_batch_size(self._h, C.byref(n), C.byref(c)), "tsp_batch_size")
+        return c.value
+
+    def set_state(self, index: int, state) -> None:
+        case = self.cases[index]
+        if _is_torch_cuda(state):
+            if tuple(state.shape) != case.shape or str(state.dtype) != "torch.float64":
+                raise ValueError(f"state must be float64 of shape {case.shape}")
+            state = state.contiguous()
+            _lib.check(_lib.lib().tsp_batch_set_state(self._h, index, C.c_void_p(state.data_ptr()), 1),
+                       "tsp_batch_set_state")
+            return
+        arr = np.ascontiguousarray(state, dtype=np.float64)
+        if arr.shape != case.shape:
+            raise ValueError(f"state must have shape {case.shape}, got {arr.shape}")
+        _lib.check(_lib.lib().tsp_batch_set_state(self._h, index, arr.ctypes.data_as(C.c_void_p), 0),
+                   "tsp_batch_set_state")
+
+    def get_state(self, index: int) -> np.ndarray:
+        arr = np.empty(self.cases[index].shape, dtype=np.float64)
+        _lib.check(_lib.lib().tsp_batch_get_state(self._h, index, arr.ctypes.data_as(C.c_void_p), 0),
+                   "tsp_batch_get_state")
+        return arr
+
+    def time(self, index: int) -> float:
+        t = C.c_double()
+        s = C.c_int64()
+        _lib.check(_lib.lib().tsp_batch_get_time(self._h, index, C.byref(t), C.byref(s)), "tsp_batch_get_time")
+        return t.value
+
+    @property
+    def steps(self) -> int:
+        t = C.c_double()
+        s = C.c_int64()
+        _lib.check(_lib.lib().tsp_batch_get_time(self._h, 0, C.byref(t), C.byref(s)), "tsp_batch_get_time")
+        return s.value
+
+    @property
+    def kernel(self) -> dict:
+        cfg = _lib.TspBatchCfg()
+        _lib.check(_lib.lib().tsp_batch_get_kernel(self._h, C.byref(cfg)), "tsp_batch_get_kernel")
+        return cfg.as_dict()
+
+    @kernel.setter
+    def kernel(self, cfg: dict) -> None:
+        cur = self.kernel
+        cur.update(cfg)
+        c = _lib.TspBatchCfg(**cur)
+        _lib.check(_lib.lib().tsp_batch_set_kernel(self._h, C.byref(c)), "tsp_batch_set_kernel")
+
+    def rk4_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_batch_rk4_steps(self._h, int(nsteps)), "tsp_batch_rk4_steps")
+
+    def graph_rk4_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_batch_graph_rk4_steps(self._h, int(nsteps)), "tsp_batch_graph_rk4_steps")
+
+    def time_rk4(self, nsteps: int, use_graph: bool = True) -> float:
+        ms = C.c_double()
+        _lib.check(_lib.lib().tsp_batch_time_rk4(self._h, int(nsteps), int(bool(use_graph)), C.byref(ms)),
+                   "tsp_batch_time_rk4")
+        return ms.value
+
+    def sync(self) -> None:
+        _lib.check(_lib.lib().tsp_batch_sync(self._h), "tsp_batch_sync")

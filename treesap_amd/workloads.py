@@ -211,8 +211,9 @@ def wave256(seed: int = 1, n: int = 256, abl: int = 16):
     return case, state, 1000
 
 
-def ensemble(seed: int = 1, count: int = 512, sides=(32, 48, 64, 96, 128)):
-    """Independent periodic (phi, Pi) instances with seeded side, stencil and KO strength; 200 steps each."""
+def ensemble_params(seed: int = 1, count: int = 512, sides=(32, 48, 64, 96, 128)):
+    """Cases and Gaussian centres of the ensemble: periodic (phi, Pi) instances with seeded side,
+    stencil width n (KO order n, ghost width n), KO strength U[0.05, 0.3] and dt = dx/4."""
     rng = SplitMix64(_seed("ensemble", seed))
     out = []
     for _ in range(count):
@@ -222,10 +223,35 @@ def ensemble(seed: int = 1, count: int = 512, sides=(32, 48, 64, 96, 128)):
         case = Case(system="phipi", n=(n, n, n), stencil_n=sn, ghost=sn, ko_order=sn, ko_strength=sigma,
                     periodic=(1, 1, 1), bc="noop", dt=0.25 * (1.0 / n))
         c = [rng.uniform(0.3, 0.7) for _ in range(3)]
+        out.append((case, c))
+    return out
+
+
+def ensemble(seed: int = 1, count: int = 512, sides=(32, 48, 64, 96, 128)):
+    """Ensemble instances with host initial data (phi = one Gaussian of width 0.1, Pi = 0); 200 steps each."""
+    out = []
+    for case, c in ensemble_params(seed, count, sides):
         state = np.zeros(case.shape)
         state[0] = gaussians(case, [1.0], [0.1], [c])
         out.append((case, state))
     return out, 200
+
+
+def gaussian_state_device(case: Case, centre, width: float = 0.1, device: str = "cuda"):
+    """The ensemble's initial state built on the device with torch (same formula as `gaussians`, whose
+    last-bit rounding of exp may differ); used where host generation of ~1e8 cells would dominate."""
+    import torch
+
+    dx = case.dx()
+    xs = [case.prob_lo[a] + (torch.arange(case.n[a], dtype=torch.float64, device=device) + 0.5) * dx[a]
+          for a in range(3)]
+    X = xs[0].view(1, 1, -1)
+    Y = xs[1].view(1, -1, 1)
+    Z = xs[2].view(-1, 1, 1)
+    r2 = (X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2
+    st = torch.zeros(case.shape, dtype=torch.float64, device=device)
+    st[0] = torch.exp(-r2 / (width * width))
+    return st
 
 
 def small_case(system: str = "phipi", n=(24, 20, 16), stencil_n: int = 2, ko_order: int = 0,
