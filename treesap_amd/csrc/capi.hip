@@ -53,6 +53,19 @@ struct tsp_solver {
 
 namespace {
 
+// Default stage-kernel configuration for a box: tall tiles where the box is
+// tall enough, and the longest z chunk (of 64, 32, 16, 8) that still gives at
+// least two workgroups per CU, so small boxes fill the device too.
+tsp_kernel_cfg default_cfg(const tsp_desc &d) {
+    tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1};
+    const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
+    for (int zc : {64, 32, 16, 8}) {
+        c.zchunk = std::min(zc, d.n[2]);
+        if (tiles * ((d.n[2] + c.zchunk - 1) / c.zchunk) >= 512) break;
+    }
+    return c;
+}
+
 double *reg_origin(const tsp_solver *s, int reg) {
     return s->base[reg] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
 }
@@ -235,7 +248,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     s->sc = s->sk * (d->n[2] + 2 * s->G);
     s->reg_elems = s->sc * s->nc;
     s->launch = lookup_launcher(d->system, d->stencil_n, d->ko_order);
-    s->cfg.zchunk = std::min(64, d->n[2]);
+    s->cfg = default_cfg(*d);
     // Geometry::CellSize = (prob_hi - prob_lo) / n_cells; grid factors as the stencils form them
     for (int a = 0; a < 3; ++a) {
         const double dx = (d->prob_hi[a] - d->prob_lo[a]) / (double)d->n[a];
