@@ -208,6 +208,43 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
 tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms);
 tsp_status tsp_batch_sync(tsp_batch *b);
 
+/* Discrete adjoint (reverse mode) of the RK4 stepping, no reference
+ * counterpart. For the linear (phi, Pi) system and a loss
+ *   J = sum_{n=1..nsteps} j_n(phi^n at the receivers)
+ * the gradient dJ/du^0 is
+ *   lambda = 0; for n = nsteps..1: lambda += dj_n/du^n; lambda = G^T lambda,
+ * with G^T the transpose of one RK4 step applied by the reverse-stage
+ * recursion mu_3 = b_3 lambda', nu_j = L^T mu_j,
+ * mu_{j-1} = b_{j-1} lambda' + a_j nu_j, lambda = lambda' + sum_j nu_j
+ * (each reverse stage is one fused stage-kernel launch on the transpose
+ * operator, with zero ghost values on non-periodic faces).
+ * An adjoint object is bound to one solver (borrowed; it must outlive it) and
+ * a fixed receiver set and step count; `receivers` holds nrec (i, j, k) valid
+ * cell indices. */
+typedef struct tsp_adjoint tsp_adjoint;
+tsp_status tsp_adjoint_create(tsp_solver *s, const int32_t *receivers, int32_t nrec, int32_t nsteps,
+                              int32_t checkpoint_every, tsp_adjoint **out);
+tsp_status tsp_adjoint_destroy(tsp_adjoint *a);
+/* Forward pass from the solver's current state: nsteps RK4 steps (graph
+ * replayed), phi recorded at the receivers after every step into the trace
+ * [nsteps][nrec], and the state stored every checkpoint_every steps (and at
+ * the end). */
+tsp_status tsp_adjoint_forward(tsp_adjoint *a);
+tsp_status tsp_adjoint_trace(tsp_adjoint *a, double *dst, int32_t on_device);
+/* Stored state of checkpoint `slot` (slot m = step m*checkpoint_every; the
+ * last slot is the final state), valid layout as tsp_get_state. */
+tsp_status tsp_adjoint_checkpoint(tsp_adjoint *a, int32_t slot, double *dst, int32_t on_device);
+/* Backward pass: dtrace[n-1][r] = dJ/dphi^n(receiver r); writes dJ/du^0 (all
+ * components, valid layout) to grad. With recompute != 0 every segment's
+ * forward states are recomputed from its checkpoint before it is reversed
+ * (the solver is left at the end of the forward pass); the linear system's
+ * transpose does not depend on them, so the default skips that work. */
+tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dtrace_on_device, double *grad,
+                                int32_t grad_on_device, int32_t recompute);
+/* One transposed RK4 step, lam_out = G^T lam_in (valid layout), for
+ * dot-product checks. */
+tsp_status tsp_adjoint_apply(tsp_adjoint *a, const double *lam_in, double *lam_out, int32_t on_device);
+
 /* Algorithmic HBM bytes of one RK4 step per valid cell (128 * ncomp). */
 int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d);
 

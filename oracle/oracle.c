@@ -611,3 +611,95 @@ void or_euler_steps(or_solver *s, int nsteps) {
         s->r_new = old;
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* discrete adjoint of one RK4 step, linear (phi, Pi) system                  */
+/* ------------------------------------------------------------------------- */
+
+/* Transpose of the decorated (phi, Pi) operator with homogeneous ghosts (zero
+ * Dirichlet values, periodic images): D_aa and the KO stencil are symmetric,
+ * the ABL term is diagonal, so L^T = [[KO - s, Lap], [I, KO - s]]. Same
+ * operation order as the forward RHS above. */
+static double adj_rhs(const or_solver *s, int r, int c, int i, int j, int k) {
+    const int n = s->d.stencil_n;
+    double v;
+    if (c == 0) {
+        double result = 0.0;
+        result += d2(s, r, n, 0, 0, i, j, k, 1);
+        result += d2(s, r, n, 1, 1, i, j, k, 1);
+        result += d2(s, r, n, 2, 2, i, j, k, 1);
+        v = result;
+    } else {
+        v = AT(s, r, i, j, k, 0);
+    }
+    if (s->d.ko_order > 0) {
+        double ko_term = 0.0;
+        ko_term += ko(s, r, s->d.ko_order, 0, i, j, k, c);
+        ko_term += ko(s, r, s->d.ko_order, 1, i, j, k, c);
+        ko_term += ko(s, r, s->d.ko_order, 2, i, j, k, c);
+        v = v + s->d.ko_strength * ko_term;
+    }
+    if (s->d.abl_thickness > 0) v = v - or_abl_sigma(s, i, j, k) * (AT(s, r, i, j, k, c) - 0.0);
+    return v;
+}
+
+/* ghost fill with homogeneous boundary values (adjoint registers) */
+static void fill_ghosts_homogeneous(or_solver *s, int r) {
+    double saved[OR_MAX_COMP];
+    int saved_bc = s->d.bc;
+    memcpy(saved, s->d.target, sizeof(saved));
+    memset(s->d.target, 0, sizeof(s->d.target));
+    s->d.bc = 1;
+    fill_ghosts(s, r);
+    memcpy(s->d.target, saved, sizeof(saved));
+    s->d.bc = saved_bc;
+}
+
+/* lambda <- G^T lambda for one RK4 step G (integrators/SingleLevelRK4Integrator.hpp:95-157):
+ *   mu_3 = b_3 lambda';  nu_j = L^T mu_j;
+ *   mu_{j-1} = b_{j-1} lambda' + a_j nu_j  (j = 3, 2, 1);
+ *   lambda = (((lambda' + nu_3) + nu_2) + nu_1) + nu_0,
+ * with b_j = dt * b_fraction[j], a_j = dt * a_fraction[j]. Uses all four
+ * registers of the solver as scratch; lambda is the valid region [c][k][j][i]. */
+int or_adjoint_step(or_solver *s, double *lam) {
+    if (s->d.system != 0) return -1;
+    const double dt = s->d.dt;
+    const int nx = s->d.n[0], ny = s->d.n[1], nz = s->d.n[2];
+    const int LP = 0, ACC = 1, MA = 2, MB = 3; /* lambda', acc, mu ping-pong */
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double v = lam[(((int64_t)c * nz + k) * ny + j) * nx + i];
+                    AT(s, LP, i, j, k, c) = v;
+                    AT(s, MA, i, j, k, c) = (dt * b_frac[3]) * v;
+                }
+    int src = MA, dst = MB;
+    for (int stage = 3; stage >= 0; --stage) {
+        fill_ghosts_homogeneous(s, src);
+        const double bprev = stage > 0 ? dt * b_frac[stage - 1] : 0.0;
+        const double astage = dt * a_frac[stage];
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int c = 0; c < 2; ++c) {
+                        double nu = adj_rhs(s, src, c, i, j, k);
+                        double lp = AT(s, LP, i, j, k, c);
+                        if (stage == 3)
+                            AT(s, ACC, i, j, k, c) = lp + nu;
+                        else
+                            AT(s, ACC, i, j, k, c) = AT(s, ACC, i, j, k, c) + nu;
+                        if (stage > 0) AT(s, dst, i, j, k, c) = bprev * lp + astage * nu;
+                    }
+        int t = src;
+        src = dst;
+        dst = t;
+    }
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    lam[(((int64_t)c * nz + k) * ny + j) * nx + i] = AT(s, ACC, i, j, k, c);
+    return 0;
+}

@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
         h.or_binom.argtypes = [C.c_int, C.c_int]
         h.or_ko_coef.restype = C.c_int64
         h.or_ko_coef.argtypes = [C.c_int, C.c_int]
+        h.or_adjoint_step.restype = C.c_int
+        h.or_adjoint_step.argtypes = [VP, DP]
         h.or_system_ncomp.restype = C.c_int
         h.or_system_ncomp.argtypes = [C.c_int]
         _lib = h
@@ -200,6 +202,14 @@ class OracleSolver:
 
     def euler_steps(self, n: int):
         lib().or_euler_steps(self._h, int(n))
+
+    def adjoint_step(self, lam: np.ndarray) -> np.ndarray:
+        """G^T lam for one RK4 step of the linear (phi, Pi) system (registers are used as scratch)."""
+        a = np.array(lam, dtype=np.float64, copy=True, order="C")
+        assert a.shape == self.shape
+        if lib().or_adjoint_step(self._h, _dp(a)) != 0:
+            raise ValueError("adjoint step needs the (phi, Pi) system")
+        return a
 
     def point_derivative(self, dirs, i, j, k, c=0):
         d = list(dirs) + [0, 0, 0]

@@ -292,3 +292,39 @@ def test_nonlinear_term_matches_direct_evaluation():
     gz = 0.3 * 2 * np.pi * np.cos(2 * np.pi * Z)
     want = rl[1] - 0.7 * st[1] * (gx ** 2 + gy ** 2 + gz ** 2)
     assert np.max(np.abs(r[1] - want)) < 1e-3 * np.max(np.abs(want))
+
+
+# ---------------------------------------------------------------------------
+# discrete adjoint of the oracle: dot-product identity <G u, v> = <u, G^T v>
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n_st,ko,periodic,abl", [(2, 0, (0, 0, 0), 3), (4, 4, (0, 0, 0), 4), (2, 3, (1, 1, 1), 0),
+                                                  (3, 3, (1, 0, 1), 0), (1, 1, (0, 1, 0), 2)])
+def test_oracle_adjoint_dot_product(n_st, ko, periodic, abl):
+    n = (14, 11, 13)
+    case = _case(n=n, stencil_n=n_st, ghost=max(n_st, ko), ko_order=ko, ko_strength=0.2 if ko else 0.0,
+                 periodic=periodic, bc="dirichlet" if not all(periodic) else "noop", abl_thickness=abl,
+                 abl_max=2.0 if abl else 0.0, dt=0.3 / 14)
+    rng = np.random.default_rng(n_st * 10 + ko)
+    fwd = orc.OracleSolver(case)
+    adj = orc.OracleSolver(case)
+    for _ in range(3):
+        u = rng.standard_normal(fwd.shape)
+        v = rng.standard_normal(fwd.shape)
+        fwd.set_state(u)
+        fwd.rk4_steps(1)
+        gu = fwd.get_state()
+        gtv = adj.adjoint_step(v)
+        lhs = float(np.sum(gu * v))
+        rhs = float(np.sum(u * gtv))
+        assert lhs == pytest.approx(rhs, rel=1e-13, abs=1e-13 * np.sqrt(np.sum(gu * gu) * np.sum(v * v)))
+
+
+def test_oracle_adjoint_is_not_the_forward():
+    """The transpose differs from the forward step where the operator is not symmetric (the check has teeth)."""
+    case = _case(n=(12, 10, 8), stencil_n=2, ghost=2, dt=0.03)
+    s = orc.OracleSolver(case)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(s.shape)
+    s.set_state(v)
+    s.rk4_steps(1)
+    assert np.max(np.abs(s.get_state() - orc.OracleSolver(case).adjoint_step(v))) > 1e-3

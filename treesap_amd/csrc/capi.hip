@@ -684,3 +684,394 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
 }
 
 } // extern "C"
+
+// ===========================================================================
+// Discrete adjoint of the RK4 stepping for the linear (phi, Pi) system
+// ===========================================================================
+//
+// J = sum_{n=1..N} j_n(phi^n at the receivers). With G the RK4 step map
+// (affine; its linear part is the same G for every step) the gradient with
+// respect to the initial state is obtained by
+//   lambda = 0;  for n = N..1:  lambda += dj_n/du^n;  lambda = G^T lambda
+// where G^T is applied by the reverse-stage recursion
+//   mu_3 = b_3 lambda';  nu_j = L^T mu_j;  mu_{j-1} = b_{j-1} lambda' + a_j nu_j;
+//   lambda = lambda' + nu_3 + nu_2 + nu_1 + nu_0
+// (b_j = dt * b_fraction[j], a_j = dt * a_fraction[j],
+// integrators/SingleLevelRK4Integrator.hpp:71-74), each reverse stage one
+// launch of the fused stage kernel on the transpose operator.
+
+struct tsp_adjoint {
+    tsp_solver *s = nullptr;
+    int nrec = 0, nsteps = 0, K = 0, nck = 0;
+    int64_t step0 = 0;  // solver step count at the start of the forward pass
+    double time0 = 0.0; // and its clock
+    int *d_rec = nullptr;     // receiver offsets of component 0 relative to the register origin
+    double *d_trace = nullptr;
+    double *d_dtrace = nullptr;
+    int *d_counter = nullptr;
+    double *d_ckpt = nullptr; // nck dense states: steps 0, K, 2K, ... and the final step
+    int64_t state_elems = 0;
+    double *abase[4] = {nullptr, nullptr, nullptr, nullptr};
+    int arole[4] = {0, 1, 2, 3};
+    tsp::LaunchFn launch = nullptr;
+    hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
+    hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
+    tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1};
+};
+
+namespace {
+
+double *adj_origin(const tsp_adjoint *a, int reg) {
+    const tsp_solver *s = a->s;
+    return a->abase[reg] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
+}
+
+int ckpt_slot_of_step(const tsp_adjoint *a, int step) {
+    if (step == a->nsteps) return a->nck - 1;
+    return step % a->K == 0 ? step / a->K : -1;
+}
+
+void drop_adjoint_graphs(tsp_adjoint *a) {
+    for (auto *g : {&a->fgraph[0], &a->fgraph[1], &a->bgraph[0], &a->bgraph[1]})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
+tsp_status adjoint_check_graphs(tsp_adjoint *a) {
+    if (std::memcmp(&a->gcfg, &a->s->cfg, sizeof(tsp_kernel_cfg)) != 0) {
+        drop_adjoint_graphs(a);
+        a->gcfg = a->s->cfg;
+    }
+    return TSP_OK;
+}
+
+tsp_status enqueue_fwd_gather(tsp_adjoint *a) {
+    tsp_solver *s = a->s;
+    tsp_status st = enqueue_rk4_step(s, nullptr);
+    if (st != TSP_OK) return st;
+    TSP_HIP(tsp::launch_gather_trace(reg_origin(s, s->role[TSP_ROLE_OLD]), a->d_rec, a->nrec, a->d_trace, a->nsteps,
+                                     a->d_counter, s->stream));
+    return TSP_OK;
+}
+
+// lambda <- G^T lambda on the adjoint registers (lambda in role OLD)
+tsp_status enqueue_adjoint_step(tsp_adjoint *a) {
+    tsp_solver *s = a->s;
+    const double dt = s->d.dt;
+    int &OLD = a->arole[TSP_ROLE_OLD], &NEW = a->arole[TSP_ROLE_NEW], &ARG = a->arole[TSP_ROLE_ARG],
+        &ARGN = a->arole[TSP_ROLE_ARG_NEXT];
+    for (int j = 3; j >= 0; --j) {
+        if (j < 3) std::swap(ARG, ARGN);
+        tsp::StageArgs g = stage_args(s, j, 0, 0, 0, -1, 0.0, j > 0 ? dt * kAFrac[j] : 0.0);
+        g.src = adj_origin(a, j == 3 ? OLD : ARG);
+        g.init = adj_origin(a, OLD);
+        g.acc = adj_origin(a, NEW);
+        g.argn = j > 0 ? adj_origin(a, ARGN) : nullptr;
+        g.src_scale = j == 3 ? dt * kBFrac[3] : 1.0;
+        g.c_init_argn = j > 0 ? dt * kBFrac[j - 1] : 0.0;
+        g.ghost_mode = 0; // adjoint ghosts: periodic images or zero (prefilled)
+        for (int c = 0; c < TSP_MAX_COMP; ++c) g.target[c] = 0.0;
+        TSP_HIP(a->launch(g, s->cfg.tile_y, s->stream));
+    }
+    std::swap(OLD, NEW);
+    return TSP_OK;
+}
+
+tsp_status enqueue_inject_adjoint(tsp_adjoint *a) {
+    TSP_HIP(tsp::launch_inject_trace(adj_origin(a, a->arole[TSP_ROLE_OLD]), a->d_rec, a->nrec, a->d_dtrace, a->nsteps,
+                                     a->d_counter, a->s->stream));
+    return enqueue_adjoint_step(a);
+}
+
+// Capture `body` twice into a graph for the current parity if it is missing.
+template <class F>
+tsp_status capture_pair(hipStream_t stream, hipGraphExec_t *slot, int *roles, F &&body) {
+    if (*slot) return TSP_OK;
+    int saved[4];
+    std::memcpy(saved, roles, sizeof(saved));
+    TSP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    tsp_status st = body();
+    if (st == TSP_OK) st = body();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    std::memcpy(roles, saved, sizeof(saved));
+    if (st != TSP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *slot = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return TSP_OK;
+}
+
+// forward steps n0 -> n1 with the receiver gather after every step (counter must point at row n0)
+tsp_status run_forward_range(tsp_adjoint *a, int n0, int n1) {
+    tsp_solver *s = a->s;
+    int n = n0;
+    while (n < n1) {
+        if (n1 - n >= 2) {
+            const int parity = (int)(s->steps & 1);
+            tsp_status st = capture_pair(s->stream, &a->fgraph[parity], s->role, [&] { return enqueue_fwd_gather(a); });
+            if (st != TSP_OK) return st;
+            TSP_HIP(hipGraphLaunch(a->fgraph[parity], s->stream));
+            advance_clock(s);
+            advance_clock(s);
+            n += 2;
+        } else {
+            tsp_status st = enqueue_fwd_gather(a);
+            if (st != TSP_OK) return st;
+            advance_clock(s);
+            n += 1;
+        }
+    }
+    return TSP_OK;
+}
+
+// reverse steps n1 -> n0 (counter must point at row n1 - 1)
+tsp_status run_backward_range(tsp_adjoint *a, int n1, int n0, int *bparity) {
+    int n = n1;
+    while (n > n0) {
+        if (n - n0 >= 2) {
+            tsp_status st = capture_pair(a->s->stream, &a->bgraph[*bparity], a->arole,
+                                         [&] { return enqueue_inject_adjoint(a); });
+            if (st != TSP_OK) return st;
+            TSP_HIP(hipGraphLaunch(a->bgraph[*bparity], a->s->stream));
+            n -= 2;
+        } else {
+            tsp_status st = enqueue_inject_adjoint(a);
+            if (st != TSP_OK) return st;
+            *bparity ^= 1;
+            n -= 1;
+        }
+    }
+    return TSP_OK;
+}
+
+tsp_status save_checkpoint(tsp_adjoint *a, int slot) {
+    tsp_solver *s = a->s;
+    TSP_HIP(tsp::launch_pack(a->d_ckpt + (int64_t)slot * a->state_elems, reg_origin(s, s->role[TSP_ROLE_OLD]), s->d.n[0],
+                             s->d.n[1], s->d.n[2], s->nc, (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
+    return TSP_OK;
+}
+
+// put checkpoint `slot` (the state `step` steps into the forward pass) back into the forward solver
+tsp_status restore_checkpoint(tsp_adjoint *a, int slot, int step, double time) {
+    tsp_solver *s = a->s;
+    // register roles are a function of the step parity (identity after an even
+    // number of steps since tsp_create / tsp_set_state)
+    const int even[4] = {0, 1, 2, 3}, odd[4] = {1, 0, 3, 2};
+    s->steps = a->step0 + step;
+    std::memcpy(s->role, (s->steps & 1) ? odd : even, sizeof(s->role));
+    s->time = time;
+    TSP_HIP(tsp::launch_unpack(reg_origin(s, s->role[TSP_ROLE_OLD]), a->d_ckpt + (int64_t)slot * a->state_elems,
+                               s->d.n[0], s->d.n[1], s->d.n[2], s->nc, (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
+    return TSP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+tsp_status tsp_adjoint_create(tsp_solver *s, const int32_t *receivers, int32_t nrec, int32_t nsteps,
+                              int32_t checkpoint_every, tsp_adjoint **out) {
+    if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
+    if (s->d.system != TSP_SYS_WAVE_PHIPI)
+        return fail(TSP_UNSUPPORTED, "the discrete adjoint covers the linear (phi,Pi) system");
+    if (nrec < 1 || !receivers) return fail(TSP_INVALID_ARG, "need at least one receiver");
+    if (nsteps < 1) return fail(TSP_INVALID_ARG, "nsteps must be >= 1");
+    if (checkpoint_every < 1) return fail(TSP_INVALID_ARG, "checkpoint_every must be >= 1");
+    tsp::LaunchFn launch = tsp::lookup_phipi_adjoint(s->d.stencil_n, s->d.ko_order);
+    if (!launch) return fail(TSP_UNSUPPORTED, "no adjoint stage kernel for this stencil width / KO order");
+    std::vector<int> off(nrec);
+    for (int r = 0; r < nrec; ++r) {
+        const int i = receivers[3 * r], j = receivers[3 * r + 1], k = receivers[3 * r + 2];
+        if (i < 0 || i >= s->d.n[0] || j < 0 || j >= s->d.n[1] || k < 0 || k >= s->d.n[2])
+            return fail(TSP_INVALID_ARG, "receiver " + std::to_string(r) + " lies outside the valid box");
+        off[r] = (int)(i + j * s->sj + k * s->sk); // component 0 (phi)
+    }
+    tsp_adjoint *a = new (std::nothrow) tsp_adjoint();
+    if (!a) return fail(TSP_OOM, "host allocation failed");
+    auto cleanup = [&](tsp_status e) {
+        tsp_adjoint_destroy(a);
+        return e;
+    };
+    a->s = s;
+    a->nrec = nrec;
+    a->nsteps = nsteps;
+    a->K = checkpoint_every;
+    a->nck = nsteps / checkpoint_every + 1 + (nsteps % checkpoint_every != 0 ? 1 : 0);
+    a->state_elems = (int64_t)s->d.n[0] * s->d.n[1] * s->d.n[2] * s->nc;
+    a->launch = launch;
+    hipError_t e = hipMalloc((void **)&a->d_rec, sizeof(int) * nrec);
+    if (e == hipSuccess) e = hipMemcpy(a->d_rec, off.data(), sizeof(int) * nrec, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_trace, sizeof(double) * nrec * (size_t)nsteps);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_dtrace, sizeof(double) * nrec * (size_t)nsteps);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_ckpt, sizeof(double) * a->state_elems * a->nck);
+    for (int r = 0; r < 4 && e == hipSuccess; ++r) {
+        e = hipMalloc((void **)&a->abase[r], sizeof(double) * s->reg_elems);
+        // adjoint registers: ghosts are zero (homogeneous boundary values) for good
+        if (e == hipSuccess) e = hipMemsetAsync(a->abase[r], 0, sizeof(double) * s->reg_elems, s->stream);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(a->d_trace, 0, sizeof(double) * nrec * (size_t)nsteps, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipErrorOutOfMemory) return cleanup(fail(TSP_OOM, "adjoint buffers do not fit in device memory"));
+    if (e != hipSuccess) return cleanup(hip_fail(e, "adjoint allocation"));
+    *out = a;
+    return TSP_OK;
+}
+
+tsp_status tsp_adjoint_destroy(tsp_adjoint *a) {
+    if (!a) return TSP_OK;
+    if (a->s && a->s->stream) (void)hipStreamSynchronize(a->s->stream);
+    drop_adjoint_graphs(a);
+    for (void *p : {(void *)a->d_rec, (void *)a->d_trace, (void *)a->d_dtrace, (void *)a->d_counter, (void *)a->d_ckpt,
+                    (void *)a->abase[0], (void *)a->abase[1], (void *)a->abase[2], (void *)a->abase[3]})
+        if (p) (void)hipFree(p);
+    delete a;
+    return TSP_OK;
+}
+
+tsp_status tsp_adjoint_forward(tsp_adjoint *a) {
+    if (!a) return fail(TSP_INVALID_ARG, "adjoint is NULL");
+    tsp_solver *s = a->s;
+    adjoint_check_graphs(a);
+    // checkpoints and trace rows count from the state at the start of the pass
+    a->step0 = s->steps;
+    a->time0 = s->time;
+    tsp_status st = save_checkpoint(a, 0);
+    if (st != TSP_OK) return st;
+    TSP_HIP(tsp::launch_set_counter(a->d_counter, 0, s->stream));
+    int n = 0;
+    while (n < a->nsteps && st == TSP_OK) {
+        const int e = std::min(a->nsteps, (n / a->K + 1) * a->K);
+        st = run_forward_range(a, n, e);
+        n = e;
+        if (st == TSP_OK) {
+            const int slot = ckpt_slot_of_step(a, n);
+            if (slot >= 0) st = save_checkpoint(a, slot);
+        }
+    }
+    return st;
+}
+
+tsp_status tsp_adjoint_trace(tsp_adjoint *a, double *dst, int32_t on_device) {
+    if (!a || !dst) return fail(TSP_INVALID_ARG, "adjoint or destination is NULL");
+    TSP_HIP(hipStreamSynchronize(a->s->stream));
+    TSP_HIP(hipMemcpy(dst, a->d_trace, sizeof(double) * a->nrec * (size_t)a->nsteps,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    return TSP_OK;
+}
+
+tsp_status tsp_adjoint_checkpoint(tsp_adjoint *a, int32_t slot, double *dst, int32_t on_device) {
+    if (!a || !dst) return fail(TSP_INVALID_ARG, "adjoint or destination is NULL");
+    if (slot < 0 || slot >= a->nck) return fail(TSP_INVALID_ARG, "checkpoint slot out of range");
+    TSP_HIP(hipStreamSynchronize(a->s->stream));
+    TSP_HIP(hipMemcpy(dst, a->d_ckpt + (int64_t)slot * a->state_elems, sizeof(double) * a->state_elems,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    return TSP_OK;
+}
+
+tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dtrace_on_device, double *grad,
+                                int32_t grad_on_device, int32_t recompute) {
+    if (!a || !dtrace || !grad) return fail(TSP_INVALID_ARG, "adjoint, dtrace or grad is NULL");
+    tsp_solver *s = a->s;
+    adjoint_check_graphs(a);
+    TSP_HIP(hipMemcpyAsync(a->d_dtrace, dtrace, sizeof(double) * a->nrec * (size_t)a->nsteps,
+                           dtrace_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+    std::memcpy(a->arole, (const int[4]){0, 1, 2, 3}, sizeof(a->arole));
+    TSP_HIP(tsp::launch_zero(a->abase[0], s->reg_elems, s->stream)); // lambda = 0 (ghosts stay zero)
+    int bparity = 0;
+    tsp_status st = TSP_OK;
+    if (!recompute) {
+        TSP_HIP(tsp::launch_set_counter(a->d_counter, a->nsteps - 1, s->stream));
+        st = run_backward_range(a, a->nsteps, 0, &bparity);
+    } else {
+        // segments from the last to the first: restore the segment's checkpoint,
+        // recompute its forward states (and receiver trace), then reverse it
+        const double saved_time = s->time;
+        const int nseg = (a->nsteps + a->K - 1) / a->K;
+        std::vector<double> seg_time(nseg + 1, a->time0);
+        for (int m = 1; m <= nseg; ++m) { // the clock at every segment start, accumulated as the steps do
+            double t = seg_time[m - 1];
+            const int len = std::min(a->K, a->nsteps - (m - 1) * a->K);
+            for (int q = 0; q < len; ++q) t += s->d.dt;
+            seg_time[m] = t;
+        }
+        for (int m = nseg - 1; m >= 0 && st == TSP_OK; --m) {
+            const int n0 = m * a->K, n1 = std::min(a->nsteps, n0 + a->K);
+            st = restore_checkpoint(a, m, n0, seg_time[m]);
+            if (st == TSP_OK) {
+                hipError_t e = tsp::launch_set_counter(a->d_counter, n0, s->stream);
+                if (e != hipSuccess) st = hip_fail(e, "set counter");
+            }
+            if (st == TSP_OK) st = run_forward_range(a, n0, n1);
+            if (st == TSP_OK) {
+                hipError_t e = tsp::launch_set_counter(a->d_counter, n1 - 1, s->stream);
+                if (e != hipSuccess) st = hip_fail(e, "set counter");
+            }
+            if (st == TSP_OK) st = run_backward_range(a, n1, n0, &bparity);
+        }
+        // leave the forward solver at the end of the forward pass
+        if (st == TSP_OK) st = restore_checkpoint(a, a->nck - 1, a->nsteps, saved_time);
+    }
+    if (st != TSP_OK) return st;
+    const int nx = s->d.n[0], ny = s->d.n[1], nz = s->d.n[2];
+    double *dst = grad;
+    double *staging = nullptr;
+    if (!grad_on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, sizeof(double) * a->state_elems));
+        dst = staging;
+    }
+    hipError_t e = tsp::launch_pack(dst, adj_origin(a, a->arole[TSP_ROLE_OLD]), nx, ny, nz, s->nc, (int)s->sj,
+                                    (int)s->sk, (int)s->sc, s->stream);
+    if (e == hipSuccess && staging)
+        e = hipMemcpyAsync(grad, staging, sizeof(double) * a->state_elems, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) return hip_fail(e, "adjoint gradient");
+    return TSP_OK;
+}
+
+tsp_status tsp_adjoint_apply(tsp_adjoint *a, const double *lam_in, double *lam_out, int32_t on_device) {
+    if (!a || !lam_in || !lam_out) return fail(TSP_INVALID_ARG, "adjoint or buffer is NULL");
+    tsp_solver *s = a->s;
+    const int nx = s->d.n[0], ny = s->d.n[1], nz = s->d.n[2];
+    const size_t bytes = sizeof(double) * a->state_elems;
+    double *staging = nullptr;
+    const double *src = lam_in;
+    if (!on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, bytes));
+        hipError_t e = hipMemcpyAsync(staging, lam_in, bytes, hipMemcpyHostToDevice, s->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(staging);
+            return hip_fail(e, "hipMemcpyAsync");
+        }
+        src = staging;
+    }
+    std::memcpy(a->arole, (const int[4]){0, 1, 2, 3}, sizeof(a->arole));
+    hipError_t e = tsp::launch_unpack(adj_origin(a, a->arole[TSP_ROLE_OLD]), src, nx, ny, nz, s->nc, (int)s->sj,
+                                      (int)s->sk, (int)s->sc, s->stream);
+    tsp_status st = e == hipSuccess ? enqueue_adjoint_step(a) : hip_fail(e, "unpack");
+    if (st == TSP_OK) {
+        double *dst = on_device ? lam_out : staging;
+        e = tsp::launch_pack(dst, adj_origin(a, a->arole[TSP_ROLE_OLD]), nx, ny, nz, s->nc, (int)s->sj, (int)s->sk,
+                             (int)s->sc, s->stream);
+        if (e == hipSuccess && !on_device) e = hipMemcpyAsync(lam_out, staging, bytes, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) st = hip_fail(e, "adjoint apply");
+    }
+    if (staging) (void)hipFree(staging);
+    return st;
+}
+
+} // extern "C"

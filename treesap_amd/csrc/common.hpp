@@ -156,4 +156,13 @@ hipError_t launch_unpack(double *reg0, const double *dense, int nx, int ny, int 
 hipError_t launch_pack(double *dense, const double *reg0, int nx, int ny, int nz, int nc, int sj, int sk, int sc,
                        hipStream_t stream);
 
+// receiver traces of the adjoint solver
+// (rows outside [0, nrows) are skipped)
+hipError_t launch_gather_trace(const double *phi, const int *rec_off, int nrec, double *trace, int nrows,
+                               int *counter, hipStream_t stream);
+hipError_t launch_inject_trace(double *lam, const int *rec_off, int nrec, const double *dtrace, int nrows,
+                               int *counter, hipStream_t stream);
+hipError_t launch_set_counter(int *counter, int v, hipStream_t stream);
+hipError_t launch_zero(double *p, long long n, hipStream_t stream);
+
 } // namespace tsp

@@ -124,3 +124,61 @@ hipError_t launch_pack(double *dense, const double *reg0, int nx, int ny, int nz
 }
 
 } // namespace tsp
+
+namespace tsp {
+
+// Receiver trace of the adjoint solver. `counter` holds the trace row; the
+// kernels read and advance it themselves so that captured graphs can be
+// replayed step after step. One workgroup; the row is read before it is
+// advanced by thread 0 after the barrier.
+__global__ __launch_bounds__(256) void gather_trace_kernel(const double *phi, const int *rec_off, int nrec,
+                                                           double *trace, int nrows, int *counter) {
+    const int row = *counter;
+    if (row >= 0 && row < nrows)
+        for (int r = threadIdx.x; r < nrec; r += blockDim.x) trace[(long long)row * nrec + r] = phi[rec_off[r]];
+    __syncthreads();
+    if (threadIdx.x == 0) *counter = row + 1;
+}
+
+// lambda[rec] += dtrace[row][r], receivers in order (several may share a cell),
+// then the row moves back by one.
+__global__ void inject_trace_kernel(double *lam, const int *rec_off, int nrec, const double *dtrace, int nrows,
+                                    int *counter) {
+    if (threadIdx.x != 0) return;
+    const int row = *counter;
+    if (row >= 0 && row < nrows)
+        for (int r = 0; r < nrec; ++r) lam[rec_off[r]] += dtrace[(long long)row * nrec + r];
+    *counter = row - 1;
+}
+
+__global__ void set_counter_kernel(int *counter, int v) { *counter = v; }
+
+__global__ __launch_bounds__(256) void zero_kernel(double *p, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+hipError_t launch_gather_trace(const double *phi, const int *rec_off, int nrec, double *trace, int nrows,
+                               int *counter, hipStream_t stream) {
+    hipLaunchKernelGGL(gather_trace_kernel, dim3(1), dim3(256), 0, stream, phi, rec_off, nrec, trace, nrows, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_inject_trace(double *lam, const int *rec_off, int nrec, const double *dtrace, int nrows,
+                               int *counter, hipStream_t stream) {
+    hipLaunchKernelGGL(inject_trace_kernel, dim3(1), dim3(64), 0, stream, lam, rec_off, nrec, dtrace, nrows, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_counter(int *counter, int v, hipStream_t stream) {
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, stream, counter, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(double *p, long long n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+} // namespace tsp

@@ -22,6 +22,13 @@ def _is_torch_cuda(x) -> bool:
     return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
 
 
+def _torch_ready(t) -> None:
+    """The library works on its own stream: finish PyTorch's pending work on `t` first."""
+    import torch
+
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 class Solver:
     def __init__(self, case: Case, stream=None):
         self.case = case
@@ -60,6 +67,7 @@ class Solver:
             if tuple(state.shape) != self.case.shape or str(state.dtype) != "torch.float64":
                 raise ValueError(f"state must be float64 of shape {self.case.shape}")
             state = state.contiguous()
+            _torch_ready(state)
             _lib.check(_lib.lib().tsp_set_state(self._h, C.c_void_p(state.data_ptr()), 1), "tsp_set_state")
             return
         arr = np.ascontiguousarray(state, dtype=np.float64)
@@ -72,6 +80,7 @@ class Solver:
         if out is not None and _is_torch_cuda(out):
             if not out.is_contiguous() or tuple(out.shape) != self.case.shape:
                 raise ValueError("out must be contiguous with the state shape")
+            _torch_ready(out)
             _lib.check(_lib.lib().tsp_get_state(self._h, C.c_void_p(out.data_ptr()), 1), "tsp_get_state")
             return out
         arr = np.empty(self.case.shape, dtype=np.float64)
@@ -195,6 +204,7 @@ class Batch:
             if tuple(state.shape) != case.shape or str(state.dtype) != "torch.float64":
                 raise ValueError(f"state must be float64 of shape {case.shape}")
             state = state.contiguous()
+            _torch_ready(state)
             _lib.check(_lib.lib().tsp_batch_set_state(self._h, index, C.c_void_p(state.data_ptr()), 1),
                        "tsp_batch_set_state")
             return

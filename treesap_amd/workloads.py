@@ -270,3 +270,24 @@ def small_case(system: str = "phipi", n=(24, 20, 16), stencil_n: int = 2, ko_ord
     for c in range(case.ncomp):
         state[c] = fourier_field(case, rng, modes=6, kmax=3, sd=0.3)
     return case, state
+
+
+def adjoint128(seed: int = 1, n: int = 128, batch: int = 16, nrec: int = 64, nsteps: int = 500, abl: int = 12):
+    """Differentiable solve: (phi, Pi) wave n^3 (n=2 stencils, no KO), 12-cell quadratic absorbing layer
+    (max 2.0, Dirichlet 0), dt = dx/4; `batch` sources (one Gaussian of width 0.1 each, centre
+    U[0.3,0.7]^3), `nrec` receivers at seeded bulk cells, target traces N(0, 0.01) (numpy PCG64 stream).
+    Returns (case, phi0 [batch, n, n, n], receivers [nrec, 3], target [batch, nsteps, nrec], nsteps)."""
+    case = Case(system="phipi", n=(n, n, n), stencil_n=2, ghost=2, ko_order=0, ko_strength=0.0,
+                periodic=(0, 0, 0), bc="dirichlet", abl_thickness=abl, abl_max=2.0, abl_profile="quadratic",
+                dt=0.25 * (1.0 / n))
+    rng = SplitMix64(_seed("adjoint128", seed))
+    phi0 = np.empty((batch, n, n, n))
+    for b in range(batch):
+        c = [rng.uniform(0.3, 0.7) for _ in range(3)]
+        phi0[b] = gaussians(case, [1.0], [0.1], [c])
+    lo, hi = abl, n - 1 - abl
+    receivers = np.array([[lo + rng.next_u64() % (hi - lo + 1) for _ in range(3)] for _ in range(nrec)],
+                         dtype=np.int32)
+    target = np.random.Generator(np.random.PCG64(_seed("adjoint128-target", seed))).normal(
+        0.0, 0.01, size=(batch, nsteps, nrec))
+    return case, phi0, receivers, target, nsteps
