@@ -132,6 +132,59 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     }
 
 
+def adjoint_leg(n: int, batch: int, nsteps: int, K: int):
+    """Differentiable solve: (phi, Pi) n^3, n=2 stencils, 12-cell ABL, `batch` sources x `nsteps` RK4 steps,
+    64 receivers; loss and gradient through the torch.autograd.Function (forward with receiver trace and
+    checkpoints every K steps, backward through the transposed RK4 stage kernels), against the plain
+    forward solve of the same sources (graph-replayed steps)."""
+    import torch
+
+    from treesap_amd import Solver
+    from treesap_amd.adjoint import ReceiverProblem, receiver_loss
+    from treesap_amd.workloads import adjoint128
+
+    case, phi0, rec, target, nsteps = adjoint128(n=n, batch=batch, nsteps=nsteps)
+    x = torch.tensor(phi0, device="cuda", requires_grad=True)
+    tgt = torch.tensor(target, device="cuda")
+    # plain forward of the same sources
+    s = Solver(case)
+    st = torch.zeros(case.shape, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(batch):
+        st[0] = x.detach()[b]
+        s.set_state(st)
+        s.graph_rk4_steps(nsteps)
+    s.sync()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    s.close()
+    prob = ReceiverProblem(case, rec, nsteps, batch=batch, checkpoint_every=K)
+    out = {}
+    for recompute in (False, True):
+        x.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = receiver_loss(x, tgt, prob, recompute)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out["recompute" if recompute else "stored_trace"] = {
+            "forward_s": t1 - t0, "backward_s": t2 - t1, "adjoint_over_forward": (t2 - t0) / t_plain}
+        gnorm = float(x.grad.norm())
+    prob.close()
+    return {
+        "workload": f"adjoint{n}: (phi,Pi) {n}^3, n=2, no KO, 12-cell ABL, {batch} sources x {nsteps} RK4 steps, "
+                    f"64 receivers, checkpoints every {K}",
+        "plain_forward_s": t_plain,
+        "loss": float(loss),
+        "grad_norm": gnorm,
+        **out,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +197,7 @@ def main() -> int:
     ap.add_argument("--kernel", default=None,
                     help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt] (skips the autotuner)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-vs-plain ensemble leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward leg")
     ap.add_argument("--ensemble-count", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -222,6 +276,15 @@ def main() -> int:
             f"{ens['plain_cell_updates_per_s']:.3e} cu/s, ratio {ens['batched_vs_plain']:.2f} "
             f"({time.perf_counter() - t0:.1f} s)")
 
+    adj = None
+    if not args.no_adjoint:
+        log("[bench] adjoint leg: 128^3 x 16 sources x 500 steps ...")
+        t0 = time.perf_counter()
+        adj = adjoint_leg(128, 16, 500, 50)
+        log(f"[bench] adjoint: plain forward {adj['plain_forward_s']:.3f} s; stored trace "
+            f"{adj['stored_trace']['adjoint_over_forward']:.2f}x; recompute "
+            f"{adj['recompute']['adjoint_over_forward']:.2f}x ({time.perf_counter() - t0:.1f} s)")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline ...")
@@ -261,6 +324,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched": ens,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
     return 0
