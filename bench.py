@@ -132,6 +132,41 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     }
 
 
+def tensor_leg(n: int, steps: int):
+    """Linearised ADM (h_ij, K_ij) workload: 12 comps at n^3, n=3 stencils with mixed second
+    derivatives, KO order 3, 12-cell ABL; graph-replayed steps."""
+    import torch
+
+    from treesap_amd import Solver
+    from treesap_amd.workloads import fourier_state_device
+
+    case = _tensor_case(n)
+    s = Solver(case)
+    s.set_state(fourier_state_device(case, "tensor192"))
+    s.graph_rk4_steps(2)
+    ms = s.time_rk4(steps, use_graph=True)
+    stage_ms, fill_ms = s.profile_rk4(2)
+    s.close()
+    cu = case.cells * steps / (ms / 1e3)
+    return {
+        "workload": f"tensor{n}: linearised ADM (h_ij, K_ij) 12 comps {n}^3, n=3, KO order 3 sigma=0.1, "
+                    f"12-cell ABL, {steps} RK4 steps timed",
+        "cell_updates_per_s": cu,
+        "algorithmic_GBs": cu * 128 * 12 / 1e9,
+        "roofline_frac": cu * 128 * 12 / 8e12,
+        "stage_ms": [x / 2 for x in stage_ms],
+        "ghost_fill_ms_per_step": fill_ms / 2,
+    }
+
+
+def _tensor_case(n: int):
+    from treesap_amd.workloads import Case
+
+    return Case(system="tensor", n=(n, n, n), stencil_n=3, ghost=3, ko_order=3, ko_strength=0.1,
+                periodic=(0, 0, 0), bc="dirichlet", abl_thickness=12, abl_max=2.0, abl_profile="quadratic",
+                dt=0.25 * (1.0 / n))
+
+
 def adjoint_leg(n: int, batch: int, nsteps: int, K: int):
     """Differentiable solve: (phi, Pi) n^3, n=2 stencils, 12-cell ABL, `batch` sources x `nsteps` RK4 steps,
     64 receivers; loss and gradient through the torch.autograd.Function (forward with receiver trace and
@@ -198,6 +233,7 @@ def main() -> int:
                     help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt] (skips the autotuner)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-vs-plain ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward leg")
+    ap.add_argument("--no-tensor", action="store_true", help="skip the 12-component tensor leg")
     ap.add_argument("--ensemble-count", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -276,6 +312,15 @@ def main() -> int:
             f"{ens['plain_cell_updates_per_s']:.3e} cu/s, ratio {ens['batched_vs_plain']:.2f} "
             f"({time.perf_counter() - t0:.1f} s)")
 
+    ten = None
+    if not args.no_tensor:
+        log("[bench] tensor leg: 192^3 x 12 comps ...")
+        t0 = time.perf_counter()
+        ten = tensor_leg(192, 20)
+        log(f"[bench] tensor: {ten['cell_updates_per_s']:.3e} cu/s, {ten['algorithmic_GBs']:.0f} GB/s algorithmic "
+            f"({100 * ten['roofline_frac']:.1f}%), stage ms {[round(x, 2) for x in ten['stage_ms']]} "
+            f"({time.perf_counter() - t0:.1f} s)")
+
     adj = None
     if not args.no_adjoint:
         log("[bench] adjoint leg: 128^3 x 16 sources x 500 steps ...")
@@ -325,6 +370,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batched": ens,
         "adjoint": adj,
+        "tensor": ten,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -64,6 +64,9 @@ CASES = [
     ("phiu", (30, 14, 18), 3, 3, (0, 0, 0), 4),
     ("nonlinear", (24, 20, 16), 2, 2, (1, 1, 1), 0),
     ("nonlinear", (66, 12, 14), 4, 0, (0, 0, 0), 3),
+    ("tensor", (20, 16, 12), 3, 3, (1, 1, 1), 0),
+    ("tensor", (24, 18, 20), 2, 0, (0, 0, 0), 4),
+    ("tensor", (17, 19, 15), 3, 3, (1, 0, 1), 3),
 ]
 
 
@@ -158,6 +161,18 @@ def test_wave256_full_size_one_step_matches_oracle():
     _say("[parity] wave256 full size, 1 step")
     case, state, _ = wave256()
     got, want = run_both(case, state, 1)
+    assert np.all(rel_l2_per_comp(got, want) <= REL_TOL)
+    assert np.array_equal(got, want)
+
+
+def test_tensor_workload_at_64_matches_oracle():
+    """The linearised ADM workload (12 comps, n=3, KO r=3, 12-cell ABL, 16 Fourier modes per
+    component) at 64^3 for 3 RK4 steps (the oracle needs about a minute per step at 192^3)."""
+    _say("[parity] tensor workload at 64^3, 3 steps")
+    from treesap_amd.workloads import tensor192
+
+    case, state, _ = tensor192(n=64)
+    got, want = run_both(case, state, 3)
     assert np.all(rel_l2_per_comp(got, want) <= REL_TOL)
     assert np.array_equal(got, want)
 

@@ -58,6 +58,7 @@ namespace {
 // least two workgroups per CU, so small boxes fill the device too.
 tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1};
+    if (d.system == TSP_SYS_TENSOR_ADM) c.ghost_mode = 1; // mixed stencils read filled edge/corner ghosts
     const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
     for (int zc : {64, 32, 16, 8}) {
         c.zchunk = std::min(zc, d.n[2]);
@@ -65,6 +66,10 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     }
     return c;
 }
+
+// The tensor system's stage kernel reads ghost memory (edges and corners for
+// its mixed stencils), so it always runs with the fill pass.
+bool fills_ghosts(const tsp_solver *s) { return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM; }
 
 double *reg_origin(const tsp_solver *s, int reg) {
     return s->base[reg] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
@@ -159,7 +164,7 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
         &ARGN = s->role[TSP_ROLE_ARG_NEXT];
     auto fill = [&](int reg, int slot) -> tsp_status {
-        if (s->cfg.ghost_mode != 1) return TSP_OK;
+        if (!fills_ghosts(s)) return TSP_OK;
         if (tm) TSP_HIP(hipEventRecord(tm->fill[slot].start, s->stream));
         TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, reg), s->stream));
         if (tm) TSP_HIP(hipEventRecord(tm->fill[slot].stop, s->stream));
@@ -187,7 +192,7 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
 
 tsp_status enqueue_euler_step(tsp_solver *s) {
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
-    if (s->cfg.ghost_mode == 1) TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
+    if (fills_ghosts(s)) TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
     const tsp::StageArgs a = stage_args(s, tsp::kEuler, OLD, OLD, NEW, -1, s->d.dt, 0.0);
     TSP_HIP(s->launch(a, s->cfg.tile_y, s->stream));
     std::swap(OLD, NEW);
@@ -532,7 +537,7 @@ tsp_status tsp_profile_rk4(tsp_solver *s, int64_t nsteps, double *stage_ms, doub
             e = hipEventElapsedTime(&ms, tm.stage[i].start, tm.stage[i].stop);
             if (e != hipSuccess) st = hip_fail(e, "hipEventElapsedTime");
             stage_ms[i] += ms;
-            if (s->cfg.ghost_mode == 1 && fill_ms && st == TSP_OK) {
+            if (fills_ghosts(s) && fill_ms && st == TSP_OK) {
                 e = hipEventElapsedTime(&ms, tm.fill[i].start, tm.fill[i].stop);
                 if (e != hipSuccess) st = hip_fail(e, "hipEventElapsedTime");
                 *fill_ms += ms;
@@ -600,6 +605,10 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     if (steps < 1) steps = 1;
     const std::string key = tune_key(s);
     tsp_kernel_cfg best = s->cfg;
+    if (s->d.system == TSP_SYS_TENSOR_ADM) { // one launch shape; nothing to tune
+        if (chosen) *chosen = best;
+        return TSP_OK;
+    }
     if (cache_lookup(key, &best)) {
         s->cfg = best;
         if (chosen) *chosen = best;

@@ -51,6 +51,7 @@ inline tsp::LaunchFn lookup_launcher(int system, int n, int r) {
     case TSP_SYS_WAVE_PHIPI: return tsp::lookup_phipi(n, r);
     case TSP_SYS_WAVE_PHIU: return tsp::lookup_phiu(n, r);
     case TSP_SYS_NONLINEAR: return tsp::lookup_nonlinear(n, r);
+    case TSP_SYS_TENSOR_ADM: return tsp::lookup_tensor(n, r);
     default: return nullptr;
     }
 }
@@ -100,7 +101,6 @@ inline tsp_status validate(const tsp_desc *d) {
     if (!d) return fail(TSP_INVALID_ARG, "descriptor is NULL");
     const int nc = system_ncomp(d->system);
     if (nc < 0) return fail(TSP_INVALID_ARG, "unknown system id " + std::to_string(d->system));
-    if (d->system == TSP_SYS_TENSOR_ADM) return fail(TSP_UNSUPPORTED, "the tensor (h,K) system has no stage kernel in this build");
     for (int a = 0; a < 3; ++a)
         if (d->n[a] < 1) return fail(TSP_INVALID_ARG, "n[" + std::to_string(a) + "] must be >= 1");
     if (d->stencil_n < 1 || d->stencil_n > 4) return fail(TSP_INVALID_ARG, "stencil_n must be in 1..4");
@@ -111,7 +111,8 @@ inline tsp_status validate(const tsp_desc *d) {
     if (!lookup_launcher(d->system, d->stencil_n, d->ko_order))
         return fail(TSP_UNSUPPORTED, "no stage kernel for stencil_n=" + std::to_string(d->stencil_n) +
                                          " ko_order=" + std::to_string(d->ko_order) +
-                                         " (supported: ko_order in {0, n, n+1}, <= 4)");
+                                         (d->system == TSP_SYS_TENSOR_ADM ? " (supported: ko_order in {0, n})"
+                                                                          : " (supported: ko_order in {0, n, n+1}, <= 4)"));
     for (int a = 0; a < 3; ++a) {
         if (!(d->prob_hi[a] > d->prob_lo[a])) return fail(TSP_INVALID_ARG, "prob_hi must exceed prob_lo");
         if (d->periodic[a] != 0 && d->periodic[a] != 1) return fail(TSP_INVALID_ARG, "periodic flags must be 0 or 1");

@@ -138,6 +138,7 @@ LaunchFn lookup_phipi(int n, int r);
 LaunchFn lookup_phipi_adjoint(int n, int r);
 LaunchFn lookup_phiu(int n, int r);
 LaunchFn lookup_nonlinear(int n, int r);
+LaunchFn lookup_tensor(int n, int r);
 
 // Ghost fill (SingleLevelIntegrator.hpp:153-163) of one register.
 struct GhostArgs {

@@ -211,6 +211,45 @@ def wave256(seed: int = 1, n: int = 256, abl: int = 16):
     return case, state, 1000
 
 
+def tensor192(seed: int = 1, n: int = 192, abl: int = 12):
+    """Linearised ADM (h_ij, K_ij), 12 comps, n^3, 6th-order stencils (n=3), KO order 3 sigma=0.1,
+    12-cell quadratic absorbing layer (max 2.0, Dirichlet 0), dt=dx/4; every component a sum of 16
+    Fourier modes (integer |k| <= 8, amplitudes N(0, 0.01)); 500 steps."""
+    case = Case(system="tensor", n=(n, n, n), stencil_n=3, ghost=3, ko_order=3, ko_strength=0.1,
+                periodic=(0, 0, 0), bc="dirichlet", abl_thickness=abl, abl_max=2.0, abl_profile="quadratic",
+                dt=0.25 * (1.0 / n))
+    rng = SplitMix64(_seed("tensor192", seed))
+    state = np.empty(case.shape)
+    for c in range(12):
+        state[c] = fourier_field(case, rng, modes=16, kmax=8, sd=0.01)
+    return case, state, 500
+
+
+def fourier_state_device(case: Case, seed_tag: str, seed: int = 1, modes: int = 16, kmax: int = 8,
+                         sd: float = 0.01, device: str = "cuda"):
+    """The tensor workload's initial state built on the device with torch: the same SplitMix64 draws
+    as `fourier_field`, evaluated with torch's cos (last-bit rounding may differ from numpy)."""
+    import torch
+
+    rng = SplitMix64(_seed(seed_tag, seed))
+    dx = case.dx()
+    xs = [case.prob_lo[a] + (torch.arange(case.n[a], dtype=torch.float64, device=device) + 0.5) * dx[a]
+          for a in range(3)]
+    X, Y, Z = xs[0].view(1, 1, -1), xs[1].view(1, -1, 1), xs[2].view(-1, 1, 1)
+    st = torch.zeros(case.shape, dtype=torch.float64, device=device)
+    for c in range(case.ncomp):
+        for _ in range(modes):
+            while True:
+                k = [int(rng.next_u64() % (2 * kmax + 1)) - kmax for _ in range(3)]
+                kk = k[0] ** 2 + k[1] ** 2 + k[2] ** 2
+                if 0 < kk <= kmax * kmax:
+                    break
+            amp = rng.normal(0.0, sd)
+            psi = rng.uniform(0.0, 2.0 * math.pi)
+            st[c] += amp * torch.cos(2.0 * math.pi * (k[0] * X + k[1] * Y + k[2] * Z) + psi)
+    return st
+
+
 def ensemble_params(seed: int = 1, count: int = 512, sides=(32, 48, 64, 96, 128)):
     """Cases and Gaussian centres of the ensemble: periodic (phi, Pi) instances with seeded side,
     stencil width n (KO order n, ghost width n), KO strength U[0.05, 0.3] and dt = dx/4."""
