@@ -86,7 +86,7 @@ GOOD = Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, dt=0.01)
     (dict(periodic=(0, 0, 0), bc="dirichlet", abl_thickness=8), _lib.INVALID_ARG, "bulk"),
     (dict(prob_hi=(0.0, 1.0, 1.0)), _lib.INVALID_ARG, "prob_hi"),
     (dict(dt=float("nan")), _lib.INVALID_ARG, "dt"),
-    (dict(system="tensor"), _lib.UNSUPPORTED, "tensor"),
+    (dict(system="tensor", ko_order=3, ghost=3), _lib.UNSUPPORTED, "ko_order in {0, n}"),
 ])
 def test_create_validation(bad, status, needle):
     if status is None:
