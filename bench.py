@@ -287,6 +287,21 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB x3 done")
 
+    # The same steps with the state handed over in host memory: upload, step, download
+    # (the rate a caller that keeps its fields on the host would see; not the headline value).
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.set_state(state)
+    s.graph_rk4_steps(args.steps)
+    host_out = s.get_state()
+    pcie_wall = time.perf_counter() - t0
+    pcie = {
+        "cell_updates_per_s": cells * args.steps / pcie_wall,
+        "wall_s": pcie_wall,
+        "note": "host numpy state uploaded, steps replayed, state downloaded to host, all inside the timed region",
+    }
+    del host_out
+
     # per-stage kernel durations (eager launches, events around each launch)
     stage_ms, fill_ms = s.profile_rk4(10)
     bytes_per_cell = s.rk4_bytes_per_cell()  # 128 * ncomp
@@ -368,6 +383,7 @@ def main() -> int:
             "stage_gbs_eager": stage_gbs,
         },
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
         "batched": ens,
         "adjoint": adj,
         "tensor": ten,
