@@ -1,0 +1,143 @@
+// SingleLevelIntegrator.hpp -- integrator on one uniform level whose state
+// lives in device memory behind the C ABI (SingleLevelIntegrator.hpp:36-193).
+//
+// Construction maps the model onto a device solver (device/model_map.hpp) and
+// allocates its registers; `block_size` is accepted for source compatibility.
+// The reference splits the level into boxes of at most block_size cells, and
+// its results do not depend on that split; here the level is one box, and
+// the ghost exchange between boxes never arises.
+// initialise() evaluates the initial condition on the host at cell centres
+// x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93) and uploads it.
+// pre_step() exports a plotfile when one is due. The FillPatch before each
+// stage (SingleLevelIntegrator.hpp:153-163) and the swap of old and new state
+// after the step happen inside the library's step.
+//
+// Additions without a reference counterpart: advance(n) runs n steps as
+// replays of a captured device graph (bit-identical to n calls of step()),
+// get_state()/set_state() move the valid state between host and device, and
+// autotune() picks the stage-kernel tiling for this shape.
+#pragma once
+
+#include "Integrator.hpp"
+#include "device/model_map.hpp"
+#include "device/solver.hpp"
+#include "util/PlotExporter.hpp"
+
+#include <cstdint>
+#include <optional>
+#include <stdexcept>
+#include <vector>
+
+namespace treesap {
+
+template <typename PDEModelType>
+    requires PDEModelConcept<PDEModelType>
+class SingleLevelIntegrator : public Integrator<PDEModelType> {
+  protected:
+    using Base = Integrator<PDEModelType>;
+    using Base::ncomp;
+    using Base::ngrow;
+    using typename Base::InitType;
+    using Base::model;
+    using Base::steps;
+    using Base::time;
+
+  public:
+    SingleLevelIntegrator(PDEModelType model_, const amrex::IntVect &block_size)
+        : Base(std::move(model_)), solver(device::make_desc(model)) {
+        for (int a = 0; a < 3; ++a)
+            if (block_size[a] < 1) throw std::invalid_argument("treesap: block_size entries must be positive");
+    }
+    ~SingleLevelIntegrator() override = default;
+
+    void configureExporter(const fs::path &path, int export_interval) {
+        exporter.emplace([this](std::vector<amrex::Real> &v) { v = get_state(); }, model.get_geometry(),
+                         model.get_comp_names(), path, export_interval);
+        interval_ = export_interval;
+    }
+
+    void initialise_impl() override {
+        const InitType init = model.get_init();
+        const amrex::Geometry &geom = model.get_geometry();
+        const auto dx = geom.CellSizeArray();
+        const auto lo = geom.ProbLoArray();
+        const amrex::Box &dom = geom.Domain();
+        const int nx = dom.length(0), ny = dom.length(1), nz = dom.length(2);
+        std::vector<amrex::Real> u((std::size_t)nx * ny * nz * ncomp);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int ia = dom.smallEnd(0) + i, ja = dom.smallEnd(1) + j, ka = dom.smallEnd(2) + k;
+                    const amrex::Real x = lo[0] + (ia + 0.5) * dx[0];
+                    const amrex::Real y = lo[1] + (ja + 0.5) * dx[1];
+                    const amrex::Real z = lo[2] + (ka + 0.5) * dx[2];
+                    static_for<ncomp>([&]<int c>() {
+                        u[(((std::size_t)c * nz + k) * ny + j) * nx + i] = init.template operator()<c>(x, y, z);
+                    });
+                }
+        solver.set_state(u.data());
+    }
+
+    // nsteps calls of step(), with the device steps between exports replayed from a captured graph
+    void advance(std::int64_t nsteps) {
+        while (nsteps > 0) {
+            std::int64_t chunk = nsteps;
+            if (exporter) {
+                ScopedTimer timer("Integrator::step() [PRE]");
+                pre_step();
+                chunk = std::min<std::int64_t>(nsteps, interval_ - (std::int64_t)(steps % interval_));
+            }
+            {
+                ScopedTimer timer("Integrator::advance()");
+                device_steps(chunk, true);
+                if (ScopedTimer::enabled) solver.sync();
+            }
+            const amrex::Real dt = model.dt_hint();
+            for (std::int64_t q = 0; q < chunk; ++q) {
+                steps++;
+                time += dt;
+            }
+            nsteps -= chunk;
+        }
+    }
+
+    std::vector<amrex::Real> get_state() const {
+        std::vector<amrex::Real> u((std::size_t)solver.cells() * ncomp);
+        solver.get_state(u.data());
+        return u;
+    }
+    void set_state(const std::vector<amrex::Real> &u) {
+        if ((std::int64_t)u.size() != solver.cells() * ncomp)
+            throw std::invalid_argument("treesap: set_state needs ncomp * cells values");
+        solver.set_state(u.data());
+    }
+    tsp_kernel_cfg autotune(int steps_per_candidate = 3) { return solver.autotune(steps_per_candidate); }
+    device::Solver &device_solver() { return solver; }
+
+  protected:
+    void pre_step() final override {
+        if (exporter) exporter->exportIfNecessary((std::int64_t)steps, time);
+    }
+    amrex::Real step_impl() final override {
+        const amrex::Real dt = computeNewState(model.dt_hint());
+        if (ScopedTimer::enabled) solver.sync();
+        return dt;
+    }
+    void post_step() final override {}
+
+    virtual amrex::Real computeNewState(amrex::Real dt_hint) = 0;
+    virtual void device_steps(std::int64_t n, bool graph) = 0;
+
+    void require_model_dt(amrex::Real dt) const {
+        if (!(dt == model.dt_hint()))
+            throw std::invalid_argument("treesap: the device solver steps with the model's dt_hint()");
+    }
+
+    mutable device::Solver solver;
+
+  private:
+    std::optional<PlotExporter> exporter;
+    std::int64_t interval_ = 1;
+};
+
+} // namespace treesap

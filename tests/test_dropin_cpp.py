@@ -1,0 +1,145 @@
+"""C++ drop-in API (include/treesap/): host checks, RHS arithmetic against the
+oracle, and whole TreESaP-style programs on the GPU against the oracle.
+
+The programs under tests/cpp are built by tests/cpp/build.py (part of the
+build entry point); a missing binary is built on first use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+import sys
+import zlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "tests" / "cpp"))
+
+import build as cpp_build  # noqa: E402  (tests/cpp/build.py)
+
+from oracle import oracle as orc  # noqa: E402
+from treesap_amd import _lib  # noqa: E402
+from treesap_amd.workloads import Case, case_from_desc  # noqa: E402
+
+
+def _run(args, **kw):
+    res = subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=300, **kw)
+    if res.returncode != 0:
+        raise AssertionError(f"{args[0]} exited {res.returncode}\nstdout:\n{res.stdout}\nstderr:\n{res.stderr}")
+    return res
+
+
+def test_host_selftest(tmp_path):
+    res = _run([cpp_build.binary("dropin_host"), "selftest", tmp_path])
+    assert "passed" in res.stdout
+
+
+# decorated registered right-hand sides, same ids as eval_case() in dropin_host.cpp
+EVAL_CASES = {
+    "phipi_n2_g3_ko3_abl4": dict(system="phipi", stencil_n=2, ghost=3, ko_order=3, ko_strength=0.1,
+                                 abl_thickness=4, abl_max=2.0),
+    "phipi_n4_g4_ko4_abl5": dict(system="phipi", stencil_n=4, ghost=4, ko_order=4, ko_strength=0.2,
+                                 abl_thickness=5, abl_max=2.0),
+    "phiu_n1_g1_ko1": dict(system="phiu", stencil_n=1, ghost=1, ko_order=1, ko_strength=0.05),
+    "phiu_n3_g3_abl3": dict(system="phiu", stencil_n=3, ghost=3, abl_thickness=3, abl_max=1.5),
+    "nonlinear_n2_g2_ko2": dict(system="nonlinear", stencil_n=2, ghost=2, ko_order=2, ko_strength=0.1, gamma=0.3),
+    "tensor_n2_g2_ko2_abl3": dict(system="tensor", stencil_n=2, ghost=2, ko_order=2, ko_strength=0.1,
+                                  abl_thickness=3, abl_max=1.0),
+    "tensor_n3_g3": dict(system="tensor", stencil_n=3, ghost=3),
+}
+
+
+@pytest.mark.parametrize("case_id", sorted(EVAL_CASES))
+@pytest.mark.parametrize("geometry", ["periodic", "dirichlet", "mixed_noop"])
+def test_registered_rhs_matches_oracle(tmp_path, case_id, geometry):
+    """The C++ functors (stencils, aliases, contractions, KO and ABL decorators)
+    evaluate bit for bit what the oracle evaluates -- the same arithmetic the
+    device kernels are tested against."""
+    n = (13, 11, 12)
+    periodic, bc = {"periodic": ((1, 1, 1), "noop"), "dirichlet": ((0, 0, 0), "dirichlet"),
+                    "mixed_noop": ((1, 0, 1), "noop")}[geometry]
+    kw = dict(EVAL_CASES[case_id])
+    if kw.get("abl_thickness", 0) and geometry == "periodic":
+        pytest.skip("absorbing layers are used on non-periodic boxes")
+    case = Case(n=n, periodic=periodic, bc=bc, dt=0.01, **kw)
+    rng = np.random.default_rng(zlib.crc32(f"{case_id}/{geometry}".encode()))
+    state = rng.uniform(-1.0, 1.0, size=case.shape)
+    o = orc.OracleSolver(case)
+    if bc == "noop":
+        # NoOp leaves ghost memory as it is: give the oracle's register zero ghosts, like the program's
+        reg = np.zeros(o.gshape)
+        G = case.ghost
+        reg[:, G:G + n[2], G:G + n[1], G:G + n[0]] = state
+        o.set_register(0, reg)
+    else:
+        o.set_state(state)
+    want = o.eval_rhs(0)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    state.astype(np.float64).tofile(fin)
+    _run([cpp_build.binary("dropin_host"), "eval", case_id, *n, *periodic, 1 if bc == "dirichlet" else 0, fin, fout])
+    got = np.fromfile(fout, dtype=np.float64).reshape(case.shape)
+    assert np.array_equal(got, want), f"max abs diff {np.max(np.abs(got - want))}"
+
+
+# ---- whole programs on the GPU ------------------------------------------------------
+
+SCENARIOS = ["first_order", "abl_ko", "nonlinear_euler", "tensor"]
+
+
+def _read_desc(path: Path) -> _lib.TspDesc:
+    raw = path.read_bytes()
+    d = _lib.TspDesc()
+    assert len(raw) == C.sizeof(d)
+    C.memmove(C.addressof(d), raw, len(raw))
+    return d
+
+
+@pytest.fixture(scope="module")
+def gpu_runs(tmp_path_factory):
+    out = tmp_path_factory.mktemp("dropin")
+    print(f"[dropin_gpu] scenarios {SCENARIOS} -> {out}", flush=True)
+    _run([cpp_build.binary("dropin_gpu"), out, *SCENARIOS])
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", SCENARIOS)
+def test_dropin_program_matches_oracle(gpu_runs, scenario):
+    d = gpu_runs / scenario
+    case = case_from_desc(_read_desc(d / "desc.bin"))
+    kind, steps, time = (d / "meta.txt").read_text().split()
+    steps = int(steps)
+    init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
+    final = np.fromfile(d / "final.bin", dtype=np.float64).reshape(case.shape)
+    o = orc.OracleSolver(case)
+    o.set_state(init)
+    (o.rk4_steps if kind == "rk4" else o.euler_steps)(steps)
+    want = o.get_state()
+    assert np.array_equal(final, want), f"{scenario}: max abs diff {np.max(np.abs(final - want))}"
+    t = 0.0
+    for _ in range(steps):
+        t += case.dt
+    assert float(time) == t
+    assert np.isfinite(final).all() and np.abs(final).max() > 0
+
+
+@pytest.mark.gpu
+def test_dropin_plotfiles(gpu_runs):
+    """configureExporter(path, 4) writes the state before steps 0, 4 and 8 of 12."""
+    d = gpu_runs / "first_order"
+    case = case_from_desc(_read_desc(d / "desc.bin"))
+    init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
+    names = sorted(p.name for p in d.iterdir() if p.name.startswith("plt"))
+    assert names == ["plt00000", "plt00004", "plt00008"]
+    o = orc.OracleSolver(case)
+    o.set_state(init)
+    o.rk4_steps(4)
+    fab = (d / "plt00004" / "Level_0" / "Cell_D_00000").read_bytes()
+    header, data = fab.split(b"\n", 1)
+    assert header.startswith(b"FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))((0,0,0) (31,31,31)")
+    assert np.array_equal(np.frombuffer(data, dtype=np.float64).reshape(case.shape), o.get_state())
+    hdr = (d / "plt00004" / "Header").read_text().splitlines()
+    assert hdr[0] == "HyperCLaw-V1.1" and hdr[1] == "4" and hdr[2:6] == ["phi", "u_0", "u_1", "u_2"]

@@ -106,6 +106,29 @@ def make_desc(case: Case) -> _lib.TspDesc:
     return d
 
 
+def case_from_desc(d: _lib.TspDesc) -> Case:
+    """The Case a descriptor describes (inverse of make_desc)."""
+    inv = lambda table, v: next(k for k, val in table.items() if val == v)  # noqa: E731
+    return Case(
+        system=inv(_lib.SYSTEMS, d.system),
+        n=tuple(int(d.n[a]) for a in range(3)),
+        stencil_n=int(d.stencil_n),
+        ghost=int(d.ghost),
+        ko_order=int(d.ko_order),
+        ko_strength=float(d.ko_strength),
+        prob_lo=tuple(float(d.prob_lo[a]) for a in range(3)),
+        prob_hi=tuple(float(d.prob_hi[a]) for a in range(3)),
+        periodic=tuple(int(d.periodic[a]) for a in range(3)),
+        bc=inv(_lib.BCS, d.bc),
+        abl_thickness=int(d.abl_thickness),
+        abl_max=float(d.abl_max),
+        abl_profile=inv(_lib.PROFILES, d.abl_profile),
+        target=tuple(float(d.target[c]) for c in range(_lib.TSP_MAX_COMP)),
+        gamma=float(d.gamma),
+        dt=float(d.dt),
+    )
+
+
 # ---------------------------------------------------------------------------
 # SplitMix64 stream
 # ---------------------------------------------------------------------------
