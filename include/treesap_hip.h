@@ -111,8 +111,10 @@ tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n);
  * (integrators/SingleLevelRK4Integrator.hpp:65-67, SingleLevelIntegrator.hpp:197-199):
  * validates the model, allocates the four registers (state_old, state_new,
  * arg, arg_next) in device memory. `stream` is a hipStream_t or NULL (the
- * library then creates its own default-priority stream). Validation happens
- * before any HIP call. */
+ * library then creates its own default-priority stream). The descriptor is
+ * validated before any HIP call; with an absorbing layer the device's
+ * per-workgroup LDS is then queried, since the stage kernels keep the layer's
+ * rate table (3 T^2 + 1 doubles) in LDS (TSP_UNSUPPORTED if it does not fit). */
 tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out);
 tsp_status tsp_destroy(tsp_solver *s);
 

@@ -18,10 +18,11 @@
 #include <vector>
 
 namespace tsp {
-hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, hipStream_t stream) {
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                hipStream_t stream) {
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kBatchVariants>([&]<int V>() {
-        if (variant == V) e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, stream);
+        if (variant == V) e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, max_abl_T, stream);
     });
     return e;
 }
@@ -37,6 +38,7 @@ struct tsp_batch {
     double *pool = nullptr;
     int64_t pool_elems = 0;
     std::vector<double *> sigma;  // per instance (null without ABL)
+    int max_abl_T = 0;            // thickest absorbing layer (sizes the kernels' LDS rate table)
     std::vector<tsp::BatchInstance> h_inst;
     tsp::BatchInstance *d_inst = nullptr;
     tsp::BatchWork *d_work = nullptr;
@@ -142,7 +144,7 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
             a.zalt = b->cfg.zalt;
             a.ghost_mode = 0;
             const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->stream));
+            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->stream));
         }
     }
     std::swap(OLD, NEW);
@@ -171,6 +173,12 @@ tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, 
             return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": no batched kernel for stencil_n=" +
                                              std::to_string(descs[i].stencil_n) +
                                              " ko_order=" + std::to_string(descs[i].ko_order));
+    }
+    {
+        int maxT = 0;
+        for (int i = 0; i < count; ++i) maxT = std::max(maxT, descs[i].abl_thickness);
+        tsp_status st = check_abl_lds(maxT);
+        if (st != TSP_OK) return st;
     }
     tsp_batch *b = new (std::nothrow) tsp_batch();
     if (!b) return fail(TSP_OOM, "host allocation failed");
@@ -229,6 +237,7 @@ tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, 
         for (int a = 0; a < 3; ++a) I.periodic[a] = d.periodic[a];
         I.dirichlet = d.bc == TSP_BC_DIRICHLET;
         I.abl_T = d.abl_thickness;
+        b->max_abl_T = std::max(b->max_abl_T, d.abl_thickness);
         I.dt = d.dt;
         for (int a = 0; a < 3; ++a) {
             const double dx = (d.prob_hi[a] - d.prob_lo[a]) / (double)d.n[a];

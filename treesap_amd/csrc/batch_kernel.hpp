@@ -58,7 +58,8 @@ template <class Sys, int N, int R, int TY> struct BatchedStage {
     using FS = FusedStage<Sys, N, R, TY>;
 
     __device__ __forceinline__ static void run(const BatchInstance *__restrict__ inst,
-                                               const BatchWork *__restrict__ work, const BatchArgs &b, double *lds) {
+                                               const BatchWork *__restrict__ work, const BatchArgs &b, double *lds,
+                                               double *sig_lds) {
         for (int w = blockIdx.x; w < b.nwork; w += gridDim.x) {
             const BatchWork wk = work[w];
             const BatchInstance &I = inst[wk.inst];
@@ -106,8 +107,9 @@ template <class Sys, int N, int R, int TY> struct BatchedStage {
             const int z0 = wk.zc * I.zchunk;
             const int z1 = min(z0 + I.zchunk, I.nz);
             __syncthreads(); // the previous item's waves may still read LDS
-            if (b.zalt && (wk.zc & 1)) FS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds);
-            else FS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds);
+            const double *sig = FS::stage_sigma(I.sigma_tab, I.abl_T, sig_lds);
+            if (b.zalt && (wk.zc & 1)) FS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+            else FS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds, sig);
         }
     }
 };
@@ -120,18 +122,22 @@ __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInsta
                                                                 const BatchWork *__restrict__ work,
                                                                 const BatchArgs b) {
     __shared__ double lds[FusedStage<Sys, N, R, TY>::LDS_DOUBLES];
-    BatchedStage<Sys, N, R, TY>::run(inst, work, b, lds);
+    extern __shared__ double sig_lds[];
+    BatchedStage<Sys, N, R, TY>::run(inst, work, b, lds, sig_lds);
 }
 
-template <class Sys, int N, int R> hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid,
+// `max_abl_T`: the thickest absorbing layer in the group (sizes the LDS rate table)
+template <class Sys, int N, int R> hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_T,
                                                             hipStream_t stream) {
+    const size_t shm = sigma_lds_bytes(max_abl_T);
     switch (tile_y) {
     case 8:
-        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 8>), dim3(grid), dim3(512), 0, stream, b.inst, b.work, b);
+        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 8>), dim3(grid), dim3(512), shm, stream, b.inst, b.work,
+                           b);
         break;
     case 16:
-        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 16>), dim3(grid), dim3(1024), 0, stream, b.inst, b.work,
-                           b);
+        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 16>), dim3(grid), dim3(1024), shm, stream, b.inst,
+                           b.work, b);
         break;
     default: return hipErrorInvalidValue;
     }
@@ -139,6 +145,7 @@ template <class Sys, int N, int R> hipError_t launch_batched(const BatchArgs &b,
 }
 
 // launch one variant group of the (phi, Pi) batch
-hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, hipStream_t stream);
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                hipStream_t stream);
 
 } // namespace tsp

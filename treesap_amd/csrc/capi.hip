@@ -241,6 +241,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     *out = nullptr;
     tsp_status st = validate(d);
     if (st != TSP_OK) return st;
+    if (d->system != TSP_SYS_TENSOR_ADM && (st = check_abl_lds(d->abl_thickness)) != TSP_OK) return st;
     tsp_solver *s = new (std::nothrow) tsp_solver();
     if (!s) return fail(TSP_OOM, "host allocation failed");
     s->d = *d;

@@ -81,6 +81,24 @@ inline std::vector<double> abl_sigma_table(const tsp_desc &d) {
     return tab;
 }
 
+// The stage kernels keep the ABL rate table in LDS beside their plane buffers
+// (at most 56 KiB: 64 x 16 tiles, two staged components, halo 4, double
+// buffered); reject layers whose table would not fit the device's LDS.
+inline tsp_status check_abl_lds(int T) {
+    if (T <= 0) return TSP_OK;
+    int dev = 0, max_lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute(MaxSharedMemoryPerBlock)");
+    const long long need = 56LL * 1024 + (long long)sizeof(double) * (3LL * T * T + 1);
+    if (need > max_lds)
+        return fail(TSP_UNSUPPORTED, "ABL thickness " + std::to_string(T) + " needs a " +
+                                         std::to_string(sizeof(double) * (3LL * T * T + 1)) +
+                                         "-byte rate table in LDS beside the 56 KiB plane buffers; the device has " +
+                                         std::to_string(max_lds) + " bytes per workgroup");
+    return TSP_OK;
+}
+
 // Register layout of one box: pitch P (multiple of 16 doubles), strides in elements.
 struct Layout {
     int G = 0, pitch = 0;

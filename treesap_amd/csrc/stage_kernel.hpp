@@ -229,20 +229,32 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     }
 
     // One 64 x TY column tile over z planes [z0, z1). `lds_raw` holds
-    // LDS_DOUBLES doubles of LDS. Every thread of the block must call this.
+    // LDS_DOUBLES doubles of LDS; `sig` is the ABL rate table by squared
+    // distance, resident in LDS (null without an ABL). Every thread of the
+    // block must call this.
     //
     // Forward (RK4 stage s / Euler, integrators/SingleLevelRK4Integrator.hpp:142-154):
     //   k = F(src);  acc = (first ? init : acc) + b*k;  argn = init + a_next*k
     // Adjoint (reverse stage j of the discrete adjoint RK4, Sys = transpose system):
     //   nu = L^T mu;  acc = (first ? lambda' : acc) + nu;  argn = b_{j-1}*lambda' + a_j*nu
+    //
+    // Memory pipeline. While plane t is computed, the x/y halo of plane t+1
+    // and the z-queue head of plane t+2 are in flight; the init/acc values of
+    // plane t are issued at the top of its step and consumed at its end. The
+    // loads that carry values into the next step are unconditional (their
+    // indices are clamped into the allocation; values beyond the chunk are
+    // never used), so each one lands in a fixed register and no copy forces
+    // an early wait. The memory counter completes in order, so every wait
+    // targets a load issued a phase earlier and leaves the newer ones in flight.
     template <bool DOWN>
     __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
-                                                    double *lds_raw) {
+                                                    double *lds_raw, const double *sig) {
         const int tid = threadIdx.x;
         const int lane = tid & 63;
         const int row = tid >> 6;
         const int gi = x0 + lane, gj = y0 + row;
         const bool active = gi < a.nx && gj < a.ny;
+        const int cnt = z1 - z0;
 
         const int stage = a.stage;
         bool is_first, write_argn, read_init;
@@ -255,13 +267,17 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             write_argn = stage < kStage3;
             read_init = write_argn && !is_first;
         }
+        const bool read_acc = !is_first;
         const int wrapx = a.ghost_mode == 0 && a.periodic[0];
         const int wrapy = a.ghost_mode == 0 && a.periodic[1];
         const int wrapz = a.ghost_mode == 0 && a.periodic[2];
         auto zoff_of = [&](int k) { return map_axis(k, a.nz, wrapz, a.G) * a.sk; };
+        auto plane = [&](int t) { return DOWN ? z1 - 1 - t : z0 + t; };          // z of step t
+        auto qplane = [&](int t) { return DOWN ? z1 - 1 - t - H : z0 + t + H; }; // queue head of step t
 
         // column offsets of this lane's own column and of its halo cells
         const int qcol = map_axis(gi, a.nx, wrapx, a.G) + map_axis(gj, a.ny, wrapy, a.G) * a.sj;
+        const int ccol = active ? gi + gj * a.sj : qcol; // centre column (clamped for lanes outside the box)
         int hcol[HPT], hlds[HPT];
         bool hok[HPT];
 #pragma unroll
@@ -279,7 +295,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                 hx = mm < H ? mm - H : TX + (mm - H);
             }
             hok[m] = h < NHALO;
-            hcol[m] = map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj;
+            hcol[m] = hok[m] ? map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj : qcol;
             hlds[m] = (hy + H) * LW + hx + H;
         }
 
@@ -292,57 +308,49 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             cxy2 = dxv * dxv + dyv * dyv;
         }
 
-        double q[NS][Q];
-        double hv[NS][HPT];
-        {
-            const int zh = zoff_of(DOWN ? z1 - 1 : z0);
+        double q[NS][Q], qn[NS], hv[NS][HPT];
+
+        auto load_halo = [&](int t) {
+            const int zh = zoff_of(plane(t));
             static_for<0, NC>([&]<int C>() {
                 if constexpr (Staging<Sys, R>::staged(C)) {
                     constexpr int sl = Staging<Sys, R>::slot(C);
 #pragma unroll
-                    for (int m = 0; m < Q; ++m) q[sl][m] = src_at<C>(a, qcol, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
-#pragma unroll
-                    for (int m = 0; m < HPT; ++m) hv[sl][m] = hok[m] ? src_at<C>(a, hcol[m], zh) : 0.0;
+                    for (int m = 0; m < HPT; ++m) hv[sl][m] = src_at<C>(a, hcol[m], zh);
                 }
             });
-        }
-
-        for (int t = 0; t < z1 - z0; ++t) {
-            const int z = DOWN ? z1 - 1 - t : z0 + t;
-            const int zn = DOWN ? z - 1 : z + 1; // next plane
-            const bool more = t + 1 < z1 - z0;
-            // prefetch the next plane of the queue and of the halo
-            double qn[NS], hvn[NS][HPT];
-            {
-                const int zq = zoff_of(DOWN ? zn - H : zn + H);
-                const int zh = zoff_of(zn);
-                static_for<0, NC>([&]<int C>() {
-                    if constexpr (Staging<Sys, R>::staged(C)) {
-                        constexpr int sl = Staging<Sys, R>::slot(C);
-                        qn[sl] = more ? src_at<C>(a, qcol, zq) : 0.0;
-#pragma unroll
-                        for (int m = 0; m < HPT; ++m) hvn[sl][m] = (more && hok[m]) ? src_at<C>(a, hcol[m], zh) : 0.0;
-                    }
-                });
-            }
-
-            // centre values of this cell: source, init and old acc
-            const int off = gi + gj * a.sj + z * a.sk;
-            double cen[NC], initc[NC], accc[NC];
+        };
+        auto load_queue = [&](int t) {
+            const int zq = zoff_of(qplane(t));
             static_for<0, NC>([&]<int C>() {
-                if constexpr (Staging<Sys, R>::staged(C)) {
-                    cen[C] = q[Staging<Sys, R>::slot(C)][H];
-                } else {
-                    double v = active ? a.src[off + C * a.sc] : 0.0;
-                    if constexpr (ADJ) v = a.src_scale * v;
-                    cen[C] = v;
-                }
-                initc[C] = (active && read_init) ? __builtin_nontemporal_load(&a.init[off + C * a.sc]) : cen[C];
-                accc[C] = (active && !is_first) ? __builtin_nontemporal_load(&a.acc[off + C * a.sc]) : 0.0;
+                if constexpr (Staging<Sys, R>::staged(C)) qn[Staging<Sys, R>::slot(C)] = src_at<C>(a, qcol, zq);
             });
+        };
 
-            // this plane into one half of the double-buffered LDS: the other
-            // half may still be read by waves finishing the previous plane
+        // prologue: queue and halo of step 0, queue head of step 1
+        static_for<0, NC>([&]<int C>() {
+            if constexpr (Staging<Sys, R>::staged(C)) {
+                constexpr int sl = Staging<Sys, R>::slot(C);
+#pragma unroll
+                for (int m = 0; m < Q; ++m)
+                    q[sl][m] = src_at<C>(a, qcol, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
+            }
+        });
+        load_halo(0);
+        load_queue(1);
+
+        for (int t = 0; t < cnt; ++t) {
+            const int z = plane(t);
+            const int off = ccol + z * a.sk;
+            // centre values of this plane: init, acc and the source of unstaged components
+            double iv[NC], av[NC], cv[NC];
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (!Staging<Sys, R>::staged(C)) cv[C] = src_at<C>(a, off, 0);
+                if (read_init) iv[C] = __builtin_nontemporal_load(&a.init[off + C * a.sc]);
+                if (read_acc) av[C] = __builtin_nontemporal_load(&a.acc[off + C * a.sc]);
+            });
+            // this plane into one half of the double-buffered LDS (the other
+            // half may still be read by waves finishing the previous plane)
             double *buf = lds_raw + (t & 1) * LDS_PLANE;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -351,14 +359,20 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                 for (int m = 0; m < HPT; ++m)
                     if (hok[m]) buf[s * LH * LW + hlds[m]] = hv[s][m];
             }
+            load_halo(t + 1);
             __syncthreads();
 
             if (active) {
+                double cen[NC];
+                static_for<0, NC>([&]<int C>() {
+                    if constexpr (Staging<Sys, R>::staged(C)) cen[C] = q[Staging<Sys, R>::slot(C)][H];
+                    else cen[C] = cv[C];
+                });
                 double sigma = 0.0;
                 if (T > 0) {
                     const int dzv = z - min(max(z, T), a.nz - 1 - T);
                     const int d2 = cxy2 + dzv * dzv;
-                    sigma = d2 == 0 ? 0.0 : a.sigma_tab[d2];
+                    sigma = d2 == 0 ? 0.0 : sig[d2];
                 }
                 const View<DOWN> v{q, reinterpret_cast<const Plane *>(buf), cen, a, lane + H, row + H};
                 static_for<0, NC>([&]<int C>() {
@@ -371,36 +385,46 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                         k = k + a.ko_strength * ko_term;
                     }
                     if (T > 0) k = k - sigma * (cen[C] - a.target[C]);
+                    const double initc = read_init ? iv[C] : cen[C];
                     double accv, argv = 0.0;
                     if constexpr (ADJ) {
-                        accv = (is_first ? initc[C] : accc[C]) + k;
-                        if (write_argn) argv = a.c_init_argn * initc[C] + a.a_next * k;
+                        accv = (is_first ? initc : av[C]) + k;
+                        if (write_argn) argv = a.c_init_argn * initc + a.a_next * k;
                     } else {
-                        if (is_first) accv = initc[C] + a.b * k;
-                        else accv = accc[C] + a.b * k;
-                        if (write_argn) argv = initc[C] + a.a_next * k;
+                        if (is_first) accv = initc + a.b * k;
+                        else accv = av[C] + a.b * k;
+                        if (write_argn) argv = initc + a.a_next * k;
                     }
                     __builtin_nontemporal_store(accv, &a.acc[off + C * a.sc]);
                     if (write_argn) __builtin_nontemporal_store(argv, &a.argn[off + C * a.sc]);
                 });
             }
 
-            // advance the queue
+            // advance the queue (its head for step t+1 was loaded during step t-1)
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
 #pragma unroll
                 for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
                 q[s][Q - 1] = qn[s];
-#pragma unroll
-                for (int m = 0; m < HPT; ++m) hv[s][m] = hvn[s][m];
             }
+            load_queue(t + 2);
         }
+    }
+
+    // Copy the ABL rate table (3 T^2 + 1 entries) into LDS; visible to the
+    // block after the first barrier of run_tile.
+    __device__ __forceinline__ static const double *stage_sigma(const double *tab, int T, double *dst) {
+        if (T <= 0) return nullptr;
+        const int nt = 3 * T * T + 1;
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) dst[i] = tab[i];
+        return dst;
     }
 
     // single box: block -> (tile x, tile y, z chunk), optionally giving each
     // XCD (blocks are dealt to the 8 XCDs round robin) a contiguous run of tiles
     __device__ __forceinline__ static void run(const StageArgs &a) {
         __shared__ double lds[LDS_DOUBLES];
+        extern __shared__ double sig_lds[];
         const int nb = gridDim.x;
         int bid = blockIdx.x;
         if (a.xcd_remap) {
@@ -413,12 +437,16 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
         const int zc = t2 / a.tiles_y;
         const int z0 = zc * a.zchunk;
         const int z1 = min(z0 + a.zchunk, a.nz);
+        const double *sig = stage_sigma(a.sigma_tab, a.abl_T, sig_lds);
         // alternate the march direction per chunk so that the planes two
         // neighbouring chunks share are read by both at about the same time
-        if (a.zalt && (zc & 1)) run_tile<true>(a, tix * TX, tiy * TY, z0, z1, lds);
-        else run_tile<false>(a, tix * TX, tiy * TY, z0, z1, lds);
+        if (a.zalt && (zc & 1)) run_tile<true>(a, tix * TX, tiy * TY, z0, z1, lds, sig);
+        else run_tile<false>(a, tix * TX, tiy * TY, z0, z1, lds, sig);
     }
 };
+
+// dynamic LDS bytes for the ABL rate table of thickness T
+inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
 template <class Sys, int N, int R, int TY, bool ADJ>
 __global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
@@ -434,11 +462,12 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     const long long nblocks = (long long)a.tiles_x * a.tiles_y * a.nzc;
     if (nblocks <= 0 || nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)nblocks);
+    const size_t shm = sigma_lds_bytes(a.abl_T);
     switch (tile_y) {
-    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), 0, stream, a); break;
-    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), 0, stream, a); break;
-    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ>), grid, dim3(512), 0, stream, a); break;
-    case 16: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 16, ADJ>), grid, dim3(1024), 0, stream, a); break;
+    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), shm, stream, a); break;
+    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a); break;
+    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ>), grid, dim3(512), shm, stream, a); break;
+    case 16: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 16, ADJ>), grid, dim3(1024), shm, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
