@@ -13,186 +13,333 @@
 //
 // Every contraction above is a left fold over m = 0, 1, 2, and the component
 // it reads, sym2(m, b) or sym2(m, m), increases with m. So one pass over the
-// six h components in increasing order can accumulate every sum in exactly
-// the reference's order: each accumulator receives at most one term per
-// component. The kernel evaluates each needed derivative of a component once
-// and adds it to the accumulators that use it, which keeps the results
-// bit-identical to evaluating the expressions term by term.
+// six h components in increasing order accumulates every sum in exactly the
+// reference's order: each accumulator receives at most one term per component.
 //
-// One thread per cell (64 x 4 x 1 blocks, x fastest), reading the source
-// register with filled ghosts (faces, edges and corners); the components are
-// visited in a loop that is not unrolled, so each component's stencils are
-// scheduled on their own.
+// Tiling. A workgroup owns a 16 x 8 x 4 block of cells (one per thread) and
+// visits the components one at a time. For each it stages the block plus an
+// H-wide halo of the (ghost-filled) source in LDS. The three unmixed second
+// derivatives and the KO terms are read from there. For the six mixed ones,
+// the inner first derivative g_i = D_i h is first evaluated at every point an
+// outer stencil reads (a "slab": the block extended by n along the outer
+// axis) and kept in LDS; the outer derivative then reads g_i. g_i at a shifted
+// point is the same expression the reference evaluates there, so the result
+// is bit-identical to the reference's nested stencil, at 2n instead of
+// (2n)^2 reads per mixed derivative.
 #include "common.hpp"
 
 namespace tsp {
 
 namespace {
 
-struct Cell {
-    const double *p; // source at (i, j, k, component 0)
-    int sj, sk, sc;
-    const StageArgs &a;
-    template <int AX> __device__ __forceinline__ int step() const { return AX == 0 ? 1 : AX == 1 ? sj : sk; }
-};
-
-// util/stencil_impl.hpp:206-222 at offset `o` from the cell
-template <int N, int AX, int C>
-__device__ __forceinline__ double d1(const Cell &x, int o) {
-    double st = 0.0;
-    static_for<1, N + 1>([&]<int S>() {
-        constexpr double w = cx::w1(N, S);
-        st += w * (x.p[o + C * x.sc + S * x.template step<AX>()] - x.p[o + C * x.sc - S * x.template step<AX>()]);
-    });
-    return x.a.gf1[AX] * st;
-}
-
-// util/stencil_impl.hpp:254-285: outer OUT of inner IN
-template <int N, int OUT, int IN, int C> __device__ __forceinline__ double d2(const Cell &x) {
-    if constexpr (OUT == IN) {
-        constexpr double w0 = cx::w2(N, 0);
-        double st = w0 * x.p[C * x.sc];
-        static_for<1, N + 1>([&]<int S>() {
-            constexpr double w = cx::w2(N, S);
-            st += w * (x.p[C * x.sc + S * x.template step<OUT>()] + x.p[C * x.sc - S * x.template step<OUT>()]);
-        });
-        return x.a.gf2[OUT] * st;
-    } else {
-        double st = 0.0;
-        static_for<1, N + 1>([&]<int S>() {
-            constexpr double w = cx::w1(N, S);
-            st += w * (d1<N, IN, C>(x, S * x.template step<OUT>()) - d1<N, IN, C>(x, -S * x.template step<OUT>()));
-        });
-        return x.a.gf1[OUT] * st;
-    }
-}
-
-// util/stencil.hpp:77-104
-template <int R, int AX, int C> __device__ __forceinline__ double ko(const Cell &x) {
-    constexpr double c0 = static_cast<double>(cx::ko_coef(R, 0));
-    constexpr double norm = cx::ko_norm(R);
-    double result = c0 * x.p[C * x.sc];
-    static_for<1, R + 1>([&]<int S>() {
-        constexpr double cs = static_cast<double>(cx::ko_coef(R, S));
-        result += cs * (x.p[C * x.sc + S * x.template step<AX>()] + x.p[C * x.sc - S * x.template step<AX>()]);
-    });
-    return result * norm * x.a.gf1[AX];
-}
-
-} // namespace
-
-// component-indexed views of the accumulator tables (runtime c in 0..5)
+// one of three values by a runtime index in 0..2, without an indexed array
 __device__ __forceinline__ double pick3(int m, double v0, double v1, double v2) {
     return m == 0 ? v0 : m == 1 ? v1 : v2;
 }
 
-template <int N, int R>
-__global__ __launch_bounds__(256) void tensor_stage_kernel(const StageArgs a) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (i >= a.nx || j >= a.ny) return;
-    const int off = i + j * a.sj + k * a.sk;
+template <int N, int R> struct TensorTile {
+    static constexpr int TX = 16, TY = 8, TZ = 4, NT = TX * TY * TZ;
+    static constexpr int H = N > R ? N : R;
+    static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, REG = RX * RY * RZ;
 
-    double sigma = 0.0;
-    const int T = a.abl_T;
-    if (T > 0) {
-        const int dx = i - min(max(i, T), a.nx - 1 - T);
-        const int dy = j - min(max(j, T), a.ny - 1 - T);
-        const int dz = k - min(max(k, T), a.nz - 1 - T);
-        const int d2v = dx * dx + dy * dy + dz * dz;
-        sigma = d2v == 0 ? 0.0 : a.sigma_tab[d2v];
+    // slab of g along axis GA extended by N along axis EA: dims and size
+    template <int EA> static constexpr int ex = TX + (EA == 0 ? 2 * N : 0);
+    template <int EA> static constexpr int ey = TY + (EA == 1 ? 2 * N : 0);
+    template <int EA> static constexpr int ez = TZ + (EA == 2 ? 2 * N : 0);
+    template <int EA> static constexpr int ssize = ex<EA> * ey<EA> * ez<EA>;
+
+    // phase 1: g_x along y (D_y D_x), g_y along x (D_x D_y), g_z along x (D_x D_z)
+    // phase 2: g_x along z (D_z D_x), g_y along z (D_z D_y), g_z along y (D_y D_z)
+    static constexpr int P1 = ssize<1> + 2 * ssize<0>;
+    static constexpr int P2 = 2 * ssize<2> + ssize<1>;
+    static constexpr int GBUF = P1 > P2 ? P1 : P2;
+    static constexpr int LDS_DOUBLES = REG + GBUF;
+
+    static __device__ __forceinline__ int ridx(int rx, int ry, int rz) { return (rz * RY + ry) * RX + rx; }
+
+    // g along axis GA at region point (rx, ry, rz) (util/stencil_impl.hpp:206-222)
+    template <int GA> static __device__ __forceinline__ double g_at(const double *u, int rx, int ry, int rz,
+                                                                      const StageArgs &a) {
+        constexpr int st = GA == 0 ? 1 : GA == 1 ? RX : RX * RY;
+        const int o = ridx(rx, ry, rz);
+        double s = 0.0;
+        static_for<1, N + 1>([&]<int S>() {
+            constexpr double w = cx::w1(N, S);
+            s += w * (u[o + S * st] - u[o - S * st]);
+        });
+        return a.gf1[GA] * s;
     }
-    const int stage = a.stage;
-    const bool is_first = stage == kStage0 || stage == kEuler;
-    const bool write_argn = stage < kStage3;
-    const bool read_init = write_argn && !is_first;
 
-    // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154)
-    auto update = [&](int c, double kv, double cenc) {
-        const int o = off + c * a.sc;
-        const double initc = read_init ? a.init[o] : cenc;
-        if (is_first) a.acc[o] = initc + a.b * kv;
-        else a.acc[o] = a.acc[o] + a.b * kv;
-        if (write_argn) a.argn[o] = initc + a.a_next * kv;
-    };
-    auto decorate = [&](int c, double kv, const Cell &x, double cenc) {
-        if constexpr (R > 0) {
-            double t = 0.0;
-            t += ko<R, 0, 0>(x);
-            t += ko<R, 1, 0>(x);
-            t += ko<R, 2, 0>(x);
-            kv = kv + a.ko_strength * t;
+    // fill slab entries [base, base + ssize<EA>) of `g` with g along GA, extended along EA
+    template <int GA, int EA> static __device__ __forceinline__ void slab_point(const double *u, double *g, int p,
+                                                                               const StageArgs &a) {
+        const int px = p % ex<EA>;
+        const int q = p / ex<EA>;
+        const int py = q % ey<EA>;
+        const int pz = q / ey<EA>;
+        const int rx = px + H - (EA == 0 ? N : 0);
+        const int ry = py + H - (EA == 1 ? N : 0);
+        const int rz = pz + H - (EA == 2 ? N : 0);
+        g[p] = g_at<GA>(u, rx, ry, rz, a);
+    }
+
+    // outer first derivative along EA of a slab extended along EA, at the thread's own cell
+    template <int EA> static __device__ __forceinline__ double outer(const double *g, int tx, int ty, int tz,
+                                                                    const StageArgs &a) {
+        constexpr int st = EA == 0 ? 1 : EA == 1 ? ex<EA> : ex<EA> * ey<EA>;
+        const int o = ((tz + (EA == 2 ? N : 0)) * ey<EA> + ty + (EA == 1 ? N : 0)) * ex<EA> + tx + (EA == 0 ? N : 0);
+        double s = 0.0;
+        static_for<1, N + 1>([&]<int S>() {
+            constexpr double w = cx::w1(N, S);
+            s += w * (g[o + S * st] - g[o - S * st]);
+        });
+        return a.gf1[EA] * s;
+    }
+
+    // unmixed second derivative along AX at the own cell (util/stencil_impl.hpp:254-268)
+    template <int AX> static __device__ __forceinline__ double d2(const double *u, int o, const StageArgs &a) {
+        constexpr int st = AX == 0 ? 1 : AX == 1 ? RX : RX * RY;
+        constexpr double w0 = cx::w2(N, 0);
+        double s = w0 * u[o];
+        static_for<1, N + 1>([&]<int S>() {
+            constexpr double w = cx::w2(N, S);
+            s += w * (u[o + S * st] + u[o - S * st]);
+        });
+        return a.gf2[AX] * s;
+    }
+
+    // Kreiss-Oliger term along AX at the own cell (util/stencil.hpp:77-104)
+    template <int AX> static __device__ __forceinline__ double ko(const double *u, int o, const StageArgs &a) {
+        constexpr int st = AX == 0 ? 1 : AX == 1 ? RX : RX * RY;
+        constexpr double c0 = static_cast<double>(cx::ko_coef(R, 0));
+        constexpr double norm = cx::ko_norm(R);
+        double result = c0 * u[o];
+        static_for<1, R + 1>([&]<int S>() {
+            constexpr double cs = static_cast<double>(cx::ko_coef(R, S));
+            result += cs * (u[o + S * st] + u[o - S * st]);
+        });
+        return result * norm * a.gf1[AX];
+    }
+
+    static __device__ void run(const StageArgs &a, double *lds) {
+        double *u = lds;
+        double *g = lds + REG;
+        const int tid = threadIdx.x;
+        const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+        const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+        // blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous
+        // run of tiles so that neighbours share an L2 for their common halo
+        const int nb = gridDim.x;
+        int bid = blockIdx.x;
+        {
+            const int per = nb / 8, rem = nb % 8, xcd = bid % 8;
+            bid = xcd * per + (xcd < rem ? xcd : rem) + bid / 8;
         }
-        if (T > 0) kv = kv - sigma * (cenc - a.target[c]);
-        return kv;
-    };
+        const int x0 = (bid % tiles_x) * TX;
+        const int y0 = ((bid / tiles_x) % tiles_y) * TY;
+        const int z0 = (bid / (tiles_x * tiles_y)) * TZ;
+        const int i = x0 + tx, j = y0 + ty, k = z0 + tz;
+        const bool active = i < a.nx && j < a.ny && k < a.nz;
+        const int off = active ? i + j * a.sj + k * a.sk : 0;
+        const int own = ridx(tx + H, ty + H, tz + H);
 
-    // f[A][B] = sum_m D_m D_A h_{mB}, tdd[A][B] = sum_m D_A D_B h_mm, lap[A][B] = sum_m D_m D_m h_AB
-    double f[3][3], tdd[3][3], lap[3][3];
-    static_for<0, 3>([&]<int A>() {
-        static_for<0, 3>([&]<int B>() {
-            f[A][B] = 0.0;
-            tdd[A][B] = 0.0;
-            lap[A][B] = 0.0;
-        });
-    });
+        const int stage = a.stage;
+        const bool is_first = stage == kStage0 || stage == kEuler;
+        const bool write_argn = stage < kStage3;
+        const bool read_init = write_argn && !is_first;
 
-    // h components in increasing order (the reference's contraction order)
-#pragma unroll 1
-    for (int c = 0; c < 6; ++c) {
-        const Cell x{a.src + off + c * a.sc, a.sj, a.sk, a.sc, a};
-        double dv[3][3];
-        static_for<0, 3>([&]<int O>() { static_for<0, 3>([&]<int I>() { dv[O][I] = d2<N, O, I, 0>(x); }); });
-        const int row = c >= 3 ? 2 : c >= 1 ? 1 : 0;
-        const int col = c - (row * (row + 1)) / 2;
-        // f[A][B] gets D_m D_A h_{mB} where sym2(m, B) == c, i.e. (m, B) is (row, col) or (col, row)
+        double sigma = 0.0;
+        const int T = a.abl_T;
+        if (T > 0 && active) {
+            const int dx = i - min(max(i, T), a.nx - 1 - T);
+            const int dy = j - min(max(j, T), a.ny - 1 - T);
+            const int dz = k - min(max(k, T), a.nz - 1 - T);
+            const int d2v = dx * dx + dy * dy + dz * dz;
+            sigma = d2v == 0 ? 0.0 : a.sigma_tab[d2v];
+        }
+
+        // The source of one component over the block plus halo (the register's
+        // ghosts are filled; indices of partial blocks are clamped into it) is
+        // loaded into registers one component ahead and written to LDS at the
+        // start of that component's pass, so its latency overlaps the previous
+        // component's arithmetic.
+        constexpr int RPT = (REG + NT - 1) / NT;
+        int roff[RPT];
+#pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            const int p = min(tid + m * NT, REG - 1);
+            const int rx = p % RX, q = p / RX, ry = q % RY, rz = q / RY;
+            const int x = min(x0 + rx - H, a.nx + a.G - 1);
+            const int y = min(y0 + ry - H, a.ny + a.G - 1);
+            const int z = min(z0 + rz - H, a.nz + a.G - 1);
+            roff[m] = x + y * a.sj + z * a.sk;
+        }
+        double pre[RPT];
+        auto load_region = [&](int c) {
+            const double *src = a.src + (long long)c * a.sc;
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) pre[m] = src[roff[m]];
+        };
+        auto stage_region = [&]() {
+#pragma unroll
+            for (int m = 0; m < RPT; ++m)
+                if (tid + m * NT < REG) u[tid + m * NT] = pre[m];
+        };
+        // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154)
+        auto update = [&](int c, double kv, double cenc, double initv, double accv) {
+            const long long o = off + (long long)c * a.sc;
+            const double initc = read_init ? initv : cenc;
+            if (is_first) a.acc[o] = initc + a.b * kv;
+            else a.acc[o] = accv + a.b * kv;
+            if (write_argn) a.argn[o] = initc + a.a_next * kv;
+        };
+        auto decorate = [&](int c, double kv, double ko_term, double cenc) {
+            if constexpr (R > 0) kv = kv + a.ko_strength * ko_term;
+            if (T > 0) kv = kv - sigma * (cenc - a.target[c]);
+            return kv;
+        };
+        auto ko_sum = [&]() {
+            double t = 0.0;
+            if constexpr (R > 0) {
+                t += ko<0>(u, own, a);
+                t += ko<1>(u, own, a);
+                t += ko<2>(u, own, a);
+            }
+            return t;
+        };
+
+        // f[A][B] = sum_m D_m D_A h_{mB}, tdd[A][B] = sum_m D_A D_B h_mm, lap[A][B] = sum_m D_m D_m h_AB
+        double f[3][3], tdd[3][3], lap[3][3];
         static_for<0, 3>([&]<int A>() {
             static_for<0, 3>([&]<int B>() {
-                if (B == col) f[A][B] += pick3(row, dv[0][A], dv[1][A], dv[2][A]);
-                else if (B == row) f[A][B] += pick3(col, dv[0][A], dv[1][A], dv[2][A]);
-                if (row == col && B <= A) tdd[A][B] += dv[A][B];
-                if (B <= A && A == row && B == col) {
-                    lap[A][B] += dv[0][0];
-                    lap[A][B] += dv[1][1];
-                    lap[A][B] += dv[2][2];
-                }
+                f[A][B] = 0.0;
+                tdd[A][B] = 0.0;
+                lap[A][B] = 0.0;
             });
         });
-        // d_t h_c = -2 K_c (+ KO, ABL)
-        const double cenc = x.p[0];
-        const double kv = decorate(c, -2.0 * x.p[6 * a.sc], x, cenc);
-        update(c, kv, cenc);
-    }
-    // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap)
+
+        // h components in increasing order (the reference's contraction order)
+        load_region(0);
 #pragma unroll 1
-    for (int c = 6; c < 12; ++c) {
-        const Cell x{a.src + off + c * a.sc, a.sj, a.sk, a.sc, a};
-        const int row = c >= 9 ? 2 : c >= 7 ? 1 : 0;
-        const int col = (c - 6) - (row * (row + 1)) / 2;
-        double fab = 0.0, fba = 0.0, tv = 0.0, lv = 0.0;
-        static_for<0, 3>([&]<int A>() {
-            static_for<0, 3>([&]<int B>() {
-                if (A == row && B == col) {
-                    fab = f[A][B];
-                    tv = tdd[A][B];
-                    lv = lap[A][B];
-                }
-                if (A == col && B == row) fba = f[A][B];
+        for (int c = 0; c < 6; ++c) {
+            __syncthreads(); // the previous component's reads of u and g are done
+            stage_region();
+            if (c < 5 || R > 0) load_region(c + 1); // next h component, or the first K component for KO
+            double kc = 0.0, initv = 0.0, accv = 0.0;
+            if (active) {
+                kc = a.src[off + (long long)(6 + c) * a.sc];
+                if (read_init) initv = a.init[off + (long long)c * a.sc];
+                if (!is_first) accv = a.acc[off + (long long)c * a.sc];
+            }
+            __syncthreads();
+
+            double dv[3][3];
+            dv[0][0] = d2<0>(u, own, a);
+            dv[1][1] = d2<1>(u, own, a);
+            dv[2][2] = d2<2>(u, own, a);
+            const double ko_term = ko_sum();
+            const double cenc = u[own];
+
+            // phase 1 slabs: g_x along y, g_y along x, g_z along x
+            constexpr int SA = ssize<1>, SB = ssize<0>;
+#pragma unroll 1
+            for (int p = tid; p < P1; p += NT) {
+                if (p < SA) slab_point<0, 1>(u, g, p, a);
+                else if (p < SA + SB) slab_point<1, 0>(u, g + SA, p - SA, a);
+                else slab_point<2, 0>(u, g + SA + SB, p - SA - SB, a);
+            }
+            __syncthreads();
+            dv[1][0] = outer<1>(g, tx, ty, tz, a);
+            dv[0][1] = outer<0>(g + SA, tx, ty, tz, a);
+            dv[0][2] = outer<0>(g + SA + SB, tx, ty, tz, a);
+            __syncthreads();
+
+            // phase 2 slabs: g_x along z, g_y along z, g_z along y
+            constexpr int SD = ssize<2>;
+#pragma unroll 1
+            for (int p = tid; p < P2; p += NT) {
+                if (p < SD) slab_point<0, 2>(u, g, p, a);
+                else if (p < 2 * SD) slab_point<1, 2>(u, g + SD, p - SD, a);
+                else slab_point<2, 1>(u, g + 2 * SD, p - 2 * SD, a);
+            }
+            __syncthreads();
+            dv[2][0] = outer<2>(g, tx, ty, tz, a);
+            dv[2][1] = outer<2>(g + SD, tx, ty, tz, a);
+            dv[1][2] = outer<1>(g + 2 * SD, tx, ty, tz, a);
+
+            const int row = c >= 3 ? 2 : c >= 1 ? 1 : 0;
+            const int col = c - (row * (row + 1)) / 2;
+            // f[A][B] gets D_m D_A h_{mB} where sym2(m, B) == c, i.e. (m, B) is (row, col) or (col, row)
+            static_for<0, 3>([&]<int A>() {
+                const double by_row = pick3(row, dv[0][A], dv[1][A], dv[2][A]);
+                const double by_col = pick3(col, dv[0][A], dv[1][A], dv[2][A]);
+                static_for<0, 3>([&]<int B>() {
+                    if (B == col) f[A][B] += by_row;
+                    else if (B == row) f[A][B] += by_col;
+                    if (row == col && B <= A) tdd[A][B] += dv[A][B];
+                    if (B <= A && A == row && B == col) {
+                        lap[A][B] += dv[0][0];
+                        lap[A][B] += dv[1][1];
+                        lap[A][B] += dv[2][2];
+                    }
+                });
             });
-        });
-        const double sym = 0.5 * (fab + fba);
-        const double cenc = x.p[0];
-        const double kv = decorate(c, sym - 0.5 * (tv + lv), x, cenc);
-        update(c, kv, cenc);
+            // d_t h_c = -2 K_c (+ KO, ABL)
+            if (active) update(c, decorate(c, -2.0 * kc, ko_term, cenc), cenc, initv, accv);
+        }
+
+        // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap)
+#pragma unroll 1
+        for (int c = 6; c < 12; ++c) {
+            double ko_term = 0.0, cenc = 0.0, initv = 0.0, accv = 0.0;
+            if constexpr (R > 0) {
+                __syncthreads();
+                stage_region();
+                if (c < 11) load_region(c + 1);
+                __syncthreads();
+                ko_term = ko_sum();
+                cenc = u[own];
+            } else {
+                if (active) cenc = a.src[off + (long long)c * a.sc];
+            }
+            if (active) {
+                if (read_init) initv = a.init[off + (long long)c * a.sc];
+                if (!is_first) accv = a.acc[off + (long long)c * a.sc];
+            }
+            const int row = c >= 9 ? 2 : c >= 7 ? 1 : 0;
+            const int col = (c - 6) - (row * (row + 1)) / 2;
+            double fab = 0.0, fba = 0.0, tv = 0.0, lv = 0.0;
+            static_for<0, 3>([&]<int A>() {
+                static_for<0, 3>([&]<int B>() {
+                    if (A == row && B == col) {
+                        fab = f[A][B];
+                        tv = tdd[A][B];
+                        lv = lap[A][B];
+                    }
+                    if (A == col && B == row) fba = f[A][B];
+                });
+            });
+            const double sym = 0.5 * (fab + fba);
+            if (active) update(c, decorate(c, sym - 0.5 * (tv + lv), ko_term, cenc), cenc, initv, accv);
+        }
     }
+};
+
+template <int N, int R>
+__global__ __launch_bounds__(512) void tensor_stage_kernel(const StageArgs a) {
+    __shared__ double lds[TensorTile<N, R>::LDS_DOUBLES];
+    TensorTile<N, R>::run(a, lds);
 }
 
+} // namespace
+
 template <int N, int R> hipError_t launch_tensor(const StageArgs &a, int tile_y, hipStream_t stream) {
-    const int ty = 4;
+    using TT = TensorTile<N, R>;
     (void)tile_y;
-    const dim3 grid((a.nx + 63) / 64, (a.ny + ty - 1) / ty, a.nz);
-    hipLaunchKernelGGL((tensor_stage_kernel<N, R>), grid, dim3(64 * ty), 0, stream, a);
+    const long long blocks = (long long)((a.nx + TT::TX - 1) / TT::TX) * ((a.ny + TT::TY - 1) / TT::TY) *
+                             ((a.nz + TT::TZ - 1) / TT::TZ);
+    if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tensor_stage_kernel<N, R>), dim3((unsigned)blocks), dim3(TT::NT), 0, stream, a);
     return hipGetLastError();
 }
 
