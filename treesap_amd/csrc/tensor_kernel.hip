@@ -57,11 +57,9 @@ template <int N, int R> struct TensorTile {
 
     static __device__ __forceinline__ int ridx(int rx, int ry, int rz) { return (rz * RY + ry) * RX + rx; }
 
-    // g along axis GA at region point (rx, ry, rz) (util/stencil_impl.hpp:206-222)
-    template <int GA> static __device__ __forceinline__ double g_at(const double *u, int rx, int ry, int rz,
-                                                                      const StageArgs &a) {
+    // g along axis GA at region offset o (util/stencil_impl.hpp:206-222)
+    template <int GA> static __device__ __forceinline__ double g_at_off(const double *u, int o, const StageArgs &a) {
         constexpr int st = GA == 0 ? 1 : GA == 1 ? RX : RX * RY;
-        const int o = ridx(rx, ry, rz);
         double s = 0.0;
         static_for<1, N + 1>([&]<int S>() {
             constexpr double w = cx::w1(N, S);
@@ -70,17 +68,26 @@ template <int N, int R> struct TensorTile {
         return a.gf1[GA] * s;
     }
 
-    // fill slab entries [base, base + ssize<EA>) of `g` with g along GA, extended along EA
-    template <int GA, int EA> static __device__ __forceinline__ void slab_point(const double *u, double *g, int p,
-                                                                               const StageArgs &a) {
-        const int px = p % ex<EA>;
-        const int q = p / ex<EA>;
-        const int py = q % ey<EA>;
-        const int pz = q / ey<EA>;
-        const int rx = px + H - (EA == 0 ? N : 0);
-        const int ry = py + H - (EA == 1 ? N : 0);
-        const int rz = pz + H - (EA == 2 ? N : 0);
-        g[p] = g_at<GA>(u, rx, ry, rz, a);
+    // The thread's share of a slab of g along GA extended by N on both sides
+    // along EA: the slab coordinate along EA runs over [0, T_EA + 2N); the
+    // thread computes coordinates t_EA, t_EA + T_EA, ... below that bound, the
+    // others equal to its own cell's. All offsets are compile-time constants
+    // from the thread's own region and slab positions.
+    template <int GA, int EA> static __device__ __forceinline__ void fill_slab(const double *u, double *g, int tx,
+                                                                              int ty, int tz, int own,
+                                                                              const StageArgs &a) {
+        constexpr int TE = EA == 0 ? TX : EA == 1 ? TY : TZ;
+        constexpr int rst = EA == 0 ? 1 : EA == 1 ? RX : RX * RY; // region stride along EA
+        constexpr int gst = EA == 0 ? 1 : EA == 1 ? ex<EA> : ex<EA> * ey<EA>; // slab stride along EA
+        const int te = EA == 0 ? tx : EA == 1 ? ty : tz;
+        const int g0 = (tz * ey<EA> + ty) * ex<EA> + tx; // slab coordinate t_EA along EA
+        const int r0 = own - N * rst;                     // the region point of slab coordinate t_EA
+        static_for<0, (TE + 2 * N + TE - 1) / TE>([&]<int K>() {
+            if (K == 0 || te + K * TE < TE + 2 * N) {
+                const int r = r0 + K * TE * rst;
+                g[g0 + K * TE * gst] = g_at_off<GA>(u, r, a);
+            }
+        });
     }
 
     // outer first derivative along EA of a slab extended along EA, at the thread's own cell
@@ -242,12 +249,9 @@ template <int N, int R> struct TensorTile {
 
             // phase 1 slabs: g_x along y, g_y along x, g_z along x
             constexpr int SA = ssize<1>, SB = ssize<0>;
-#pragma unroll 1
-            for (int p = tid; p < P1; p += NT) {
-                if (p < SA) slab_point<0, 1>(u, g, p, a);
-                else if (p < SA + SB) slab_point<1, 0>(u, g + SA, p - SA, a);
-                else slab_point<2, 0>(u, g + SA + SB, p - SA - SB, a);
-            }
+            fill_slab<0, 1>(u, g, tx, ty, tz, own, a);
+            fill_slab<1, 0>(u, g + SA, tx, ty, tz, own, a);
+            fill_slab<2, 0>(u, g + SA + SB, tx, ty, tz, own, a);
             __syncthreads();
             dv[1][0] = outer<1>(g, tx, ty, tz, a);
             dv[0][1] = outer<0>(g + SA, tx, ty, tz, a);
@@ -256,12 +260,9 @@ template <int N, int R> struct TensorTile {
 
             // phase 2 slabs: g_x along z, g_y along z, g_z along y
             constexpr int SD = ssize<2>;
-#pragma unroll 1
-            for (int p = tid; p < P2; p += NT) {
-                if (p < SD) slab_point<0, 2>(u, g, p, a);
-                else if (p < 2 * SD) slab_point<1, 2>(u, g + SD, p - SD, a);
-                else slab_point<2, 1>(u, g + 2 * SD, p - 2 * SD, a);
-            }
+            fill_slab<0, 2>(u, g, tx, ty, tz, own, a);
+            fill_slab<1, 2>(u, g + SD, tx, ty, tz, own, a);
+            fill_slab<2, 1>(u, g + 2 * SD, tx, ty, tz, own, a);
             __syncthreads();
             dv[2][0] = outer<2>(g, tx, ty, tz, a);
             dv[2][1] = outer<2>(g + SD, tx, ty, tz, a);
