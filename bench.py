@@ -37,14 +37,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per stage launch measured with rocprofv3 counters, if a summary was committed."""
+def traffic_from_profiles(cfg: dict):
+    """HBM bytes per stage launch of stage-kernel configuration `cfg`, measured with rocprofv3
+    counters (committed summary), or None if that configuration was not profiled."""
     p = ROOT / "profiles" / "fused_stage_wave256_traffic.json"
     if not p.exists():
         return None
+    key = ",".join(str(cfg[k]) for k in ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt"))
     try:
-        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
-    except (ValueError, OSError):
+        return json.loads(p.read_text())["configs"].get(key, {}).get("hbm_bytes_per_launch")
+    except (ValueError, OSError, KeyError):
         return None
 
 
@@ -92,8 +94,8 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     torch.cuda.synchronize()
     cfgs = [b.kernel]
     if tune:
-        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=1) for ty in (8, 16) for zc in (0, 64)
-                for gb in (0, 512)]
+        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=za) for ty in (8, 16) for zc in (0, 64, 128)
+                for gb in (0, 512, 1024) for za in (0, 1)]
     best = None
     for cfg in cfgs:
         b.kernel = cfg
@@ -377,7 +379,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
+            "traffic": traffic_from_profiles(cfg),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "mean_stage_launch_us": mean_launch_s * 1e6,
             "stage_gbs_eager": stage_gbs,
