@@ -216,7 +216,7 @@ def adjoint_leg(n: int, batch: int, nsteps: int, K: int):
         "workload": f"adjoint{n}: (phi,Pi) {n}^3, n=2, no KO, 12-cell ABL, {batch} sources x {nsteps} RK4 steps, "
                     f"64 receivers, checkpoints every {K}",
         "plain_forward_s": t_plain,
-        "loss": float(loss),
+        "loss": float(loss.detach()),
         "grad_norm": gnorm,
         **out,
     }
