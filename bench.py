@@ -31,6 +31,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_FIELDS = ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt", "halo_ring")
 
 
 def log(*a):
@@ -43,7 +44,7 @@ def traffic_from_profiles(cfg: dict):
     p = ROOT / "profiles" / "fused_stage_wave256_traffic.json"
     if not p.exists():
         return None
-    key = ",".join(str(cfg[k]) for k in ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt"))
+    key = ",".join(str(cfg[k]) for k in KERNEL_FIELDS)
     try:
         return json.loads(p.read_text())["configs"].get(key, {}).get("hbm_bytes_per_launch")
     except (ValueError, OSError, KeyError):
@@ -232,7 +233,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--kernel", default=None,
-                    help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt] (skips the autotuner)")
+                    help="fixed stage-kernel config ghost_mode,tile_y,zchunk,xcd_remap[,zalt[,halo_ring]] (skips the autotuner)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-vs-plain ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward leg")
     ap.add_argument("--no-tensor", action="store_true", help="skip the 12-component tensor leg")
@@ -261,7 +262,7 @@ def main() -> int:
     s.set_state(state)
     if args.kernel:
         vals = [int(v) for v in args.kernel.split(",")]
-        s.kernel = dict(zip(("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt"), vals))
+        s.kernel = dict(zip(KERNEL_FIELDS, vals))
     elif not args.no_tune:
         t0 = time.perf_counter()
         cfg = s.autotune(3)

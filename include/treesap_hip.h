@@ -96,6 +96,8 @@ typedef struct tsp_kernel_cfg {
     int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
     int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
     int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together) */
+    int32_t halo_ring;  /* 1: tile_y 8/16 load each plane's x/y halo with its z-queue head and keep
+                           it in LDS until use (falls back to 0 where the LDS does not fit) */
 } tsp_kernel_cfg;
 
 typedef struct tsp_solver tsp_solver;

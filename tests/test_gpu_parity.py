@@ -93,10 +93,12 @@ def test_euler_matches_oracle(system, n, sn, ko, per, abl):
 
 
 TILE_CFGS = [
-    {"ghost_mode": 0, "tile_y": 2, "zchunk": 3, "xcd_remap": 0, "zalt": 1},
-    {"ghost_mode": 0, "tile_y": 16, "zchunk": 7, "xcd_remap": 1, "zalt": 1},
-    {"ghost_mode": 1, "tile_y": 4, "zchunk": 5, "xcd_remap": 1, "zalt": 1},
-    {"ghost_mode": 1, "tile_y": 8, "zchunk": 1, "xcd_remap": 0, "zalt": 0},
+    {"ghost_mode": 0, "tile_y": 2, "zchunk": 3, "xcd_remap": 0, "zalt": 1, "halo_ring": 0},
+    {"ghost_mode": 0, "tile_y": 16, "zchunk": 7, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
+    {"ghost_mode": 0, "tile_y": 16, "zchunk": 9, "xcd_remap": 0, "zalt": 0, "halo_ring": 0},
+    {"ghost_mode": 1, "tile_y": 4, "zchunk": 5, "xcd_remap": 1, "zalt": 1, "halo_ring": 0},
+    {"ghost_mode": 1, "tile_y": 8, "zchunk": 1, "xcd_remap": 0, "zalt": 0, "halo_ring": 1},
+    {"ghost_mode": 0, "tile_y": 8, "zchunk": 2, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
 ]
 
 

@@ -59,6 +59,7 @@ class TspKernelCfg(C.Structure):
         ("zchunk", C.c_int32),
         ("xcd_remap", C.c_int32),
         ("zalt", C.c_int32),
+        ("halo_ring", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

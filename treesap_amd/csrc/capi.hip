@@ -42,7 +42,7 @@ struct tsp_solver {
     bool own_stream = false;
     double *sigma_tab = nullptr;
     tsp::LaunchFn launch = nullptr;
-    tsp_kernel_cfg cfg{0, 8, 64, 1, 1};
+    tsp_kernel_cfg cfg{0, 8, 64, 1, 1, 1};
     double time = 0.0;
     int64_t steps = 0;
     double gf1[3]{}, gf2[3]{};
@@ -57,7 +57,7 @@ namespace {
 // tall enough, and the longest z chunk (of 64, 32, 16, 8) that still gives at
 // least two workgroups per CU, so small boxes fill the device too.
 tsp_kernel_cfg default_cfg(const tsp_desc &d) {
-    tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1};
+    tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1};
     if (d.system == TSP_SYS_TENSOR_ADM) c.ghost_mode = 1; // mixed stencils read filled edge/corner ghosts
     const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
     for (int zc : {64, 32, 16, 8}) {
@@ -83,6 +83,7 @@ tsp_status validate_cfg(const tsp_kernel_cfg *c) {
     if (c->zchunk < 1) return fail(TSP_INVALID_ARG, "zchunk must be >= 1");
     if (c->xcd_remap != 0 && c->xcd_remap != 1) return fail(TSP_INVALID_ARG, "xcd_remap must be 0 or 1");
     if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
+    if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
     return TSP_OK;
 }
 
@@ -134,6 +135,7 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     a.zchunk = s->cfg.zchunk;
     a.xcd_remap = s->cfg.xcd_remap;
     a.zalt = s->cfg.zalt;
+    a.halo_ring = s->cfg.halo_ring;
     a.b = b;
     a.a_next = a_next;
     for (int i = 0; i < 3; ++i) {
@@ -579,8 +581,8 @@ bool cache_lookup(const std::string &key, tsp_kernel_cfg *cfg) {
         char k[256];
         tsp_kernel_cfg c{};
         double ms = 0;
-        if (std::sscanf(line, "%255s %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap, &c.zalt,
-                        &ms) == 7 &&
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
+                        &c.zalt, &c.halo_ring, &ms) == 8 &&
             key == k && validate_cfg(&c) == TSP_OK) {
             *cfg = c; // the last matching line wins
             found = true;
@@ -595,7 +597,8 @@ void cache_store(const std::string &key, const tsp_kernel_cfg &c, double ms) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap, c.zalt, ms);
+    std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap, c.zalt,
+                 c.halo_ring, ms);
     std::fclose(f);
 }
 
@@ -636,13 +639,15 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     for (int z : {8, 16, 32, 64, 128, 256})
         if (z < s->d.n[2]) zchunks.push_back(z);
     zchunks.push_back(s->d.n[2]);
-    // candidates: every tile height, z chunk, XCD mapping and z-march order
-    // with virtual ghosts, then the best geometry with the fill-pass mode
+    // candidates: every tile height, z chunk, XCD mapping, z-march order and
+    // (for the tile heights that have one) halo ring with virtual ghosts, then
+    // the best geometry with the fill-pass mode
     std::vector<tsp_kernel_cfg> cands;
     for (int ty : {2, 4, 8, 16})
         for (int zc : zchunks)
             for (int xr = 0; xr < 2; ++xr)
-                for (int za = 0; za < 2; ++za) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za});
+                for (int za = 0; za < 2; ++za)
+                    for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr});
     double best_ms = 1e300;
     auto time_cfg = [&](const tsp_kernel_cfg &c) -> tsp_status {
         s->cfg = c;
@@ -726,7 +731,7 @@ struct tsp_adjoint {
     tsp::LaunchFn launch = nullptr;
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
-    tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1};
+    tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1, -1};
 };
 
 namespace {

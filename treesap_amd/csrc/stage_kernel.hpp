@@ -180,7 +180,12 @@ __device__ __forceinline__ int map_axis(int i, int n, int wrap, int G) {
     return i;
 }
 
-template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
+// RS > 0 (halo ring): the x/y halo of a plane is loaded RS+1 steps ahead of
+// its use, about when the neighbouring tiles load those cells as their own
+// z-queue heads (H+2 steps ahead) and L2 still holds them, and parked in an
+// LDS ring of RS slots until the plane is computed. Each thread stores and
+// reloads only its own halo cells, so the ring needs no barrier.
+template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct FusedStage {
     static constexpr int TX = 64;
     static constexpr int H = N > R ? N : R;
     static constexpr int NC = Sys::NC;
@@ -193,6 +198,10 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     static constexpr int Q = 2 * H + 1;
     static constexpr int LDS_PLANE = NS * LH * LW;
     static constexpr int LDS_DOUBLES = 2 * LDS_PLANE; // double-buffered plane
+    static constexpr int RING_DOUBLES = RS * NS * NHALO;
+    static constexpr int LDS_TOTAL = LDS_DOUBLES + RING_DOUBLES;
+    // ring variants are built where planes + ring stay within 120 KiB of LDS
+    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 120 * 1024;
 
     using Plane = double[LH][LW];
 
@@ -255,6 +264,9 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
         const int gi = x0 + lane, gj = y0 + row;
         const bool active = gi < a.nx && gj < a.ny;
         const int cnt = z1 - z0;
+        // prefetches for steps past the chunk are never consumed: they reread
+        // the last step's planes, which the caches hold, instead of new ones
+        const int tlast = cnt - 1;
 
         const int stage = a.stage;
         bool is_first, write_argn, read_init;
@@ -327,7 +339,12 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             });
         };
 
-        // prologue: queue and halo of step 0, queue head of step 1
+        // halo slot m of component slot s for ring slot r (RS > 0 only)
+        double *ring = lds_raw + LDS_DOUBLES;
+        auto ring_at = [&](int r, int s, int m) -> double & { return ring[(r * NS + s) * NHALO + tid + m * NT]; };
+
+        // prologue: queue and halo of step 0, queue head of step 1; with the
+        // ring, the halos of steps 0..RS-1 parked and that of step RS in flight
         static_for<0, NC>([&]<int C>() {
             if constexpr (Staging<Sys, R>::staged(C)) {
                 constexpr int sl = Staging<Sys, R>::slot(C);
@@ -336,8 +353,22 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                     q[sl][m] = src_at<C>(a, qcol, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
             }
         });
-        load_halo(0);
-        load_queue(1);
+        if constexpr (RS > 0) {
+#pragma unroll
+            for (int p = 0; p < RS; ++p) {
+                load_halo(min(p, tlast));
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int m = 0; m < HPT; ++m)
+                        if (hok[m]) ring_at(p, s, m) = hv[s][m];
+            }
+            load_halo(min(RS, tlast));
+        } else {
+            load_halo(0);
+        }
+        load_queue(min(1, tlast));
+        int rslot = 0; // ring slot of step t: t % RS
 
         for (int t = 0; t < cnt; ++t) {
             const int z = plane(t);
@@ -356,10 +387,26 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
             for (int s = 0; s < NS; ++s) {
                 buf[s * LH * LW + (row + H) * LW + lane + H] = q[s][H];
 #pragma unroll
-                for (int m = 0; m < HPT; ++m)
-                    if (hok[m]) buf[s * LH * LW + hlds[m]] = hv[s][m];
+                for (int m = 0; m < HPT; ++m) {
+                    if (hok[m]) {
+                        if constexpr (RS > 0) {
+                            // the slot of step t takes the halo of step t+RS
+                            double &r = ring_at(rslot, s, m);
+                            const double hvt = r;
+                            r = hv[s][m];
+                            buf[s * LH * LW + hlds[m]] = hvt;
+                        } else {
+                            buf[s * LH * LW + hlds[m]] = hv[s][m];
+                        }
+                    }
+                }
             }
-            load_halo(t + 1);
+            if constexpr (RS > 0) {
+                load_halo(min(t + RS + 1, tlast));
+                rslot = rslot == RS - 1 ? 0 : rslot + 1;
+            } else {
+                load_halo(min(t + 1, tlast));
+            }
             __syncthreads();
 
             if (active) {
@@ -407,7 +454,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
                 for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
                 q[s][Q - 1] = qn[s];
             }
-            load_queue(t + 2);
+            load_queue(min(t + 2, tlast));
         }
     }
 
@@ -423,7 +470,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
     // single box: block -> (tile x, tile y, z chunk), optionally giving each
     // XCD (blocks are dealt to the 8 XCDs round robin) a contiguous run of tiles
     __device__ __forceinline__ static void run(const StageArgs &a) {
-        __shared__ double lds[LDS_DOUBLES];
+        __shared__ double lds[LDS_TOTAL];
         extern __shared__ double sig_lds[];
         const int nb = gridDim.x;
         int bid = blockIdx.x;
@@ -448,9 +495,44 @@ template <class Sys, int N, int R, int TY, bool ADJ = false> struct FusedStage {
 // dynamic LDS bytes for the ABL rate table of thickness T
 inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
-template <class Sys, int N, int R, int TY, bool ADJ>
+template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0>
 __global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
-    FusedStage<Sys, N, R, TY, ADJ>::run(a);
+    FusedStage<Sys, N, R, TY, ADJ, RS>::run(a);
+}
+
+// Ring slots of the halo-ring variant by tile height: H+1 for 64x16 tiles (one
+// block per CU), H for 64x8 tiles (so that two blocks share a CU's LDS).
+template <int TY, int H> constexpr int ring_slots() { return TY == 16 ? H + 1 : TY == 8 ? H : 0; }
+
+// LDS a block may hold (static + dynamic), queried once per device
+inline size_t max_block_lds() {
+    static thread_local int dev_cached = -1;
+    static thread_local size_t bytes = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (dev != dev_cached) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 0;
+        bytes = (size_t)v;
+        dev_cached = dev;
+    }
+    return bytes;
+}
+
+// Tile heights with a halo-ring variant: use it when asked for and when its
+// LDS (static + the dynamic ABL table) fits a block on this device.
+template <class Sys, int N, int R, int TY, bool ADJ>
+hipError_t launch_tile(const StageArgs &a, dim3 grid, size_t shm, hipStream_t stream) {
+    constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY, ADJ>::H>();
+    using Ring = FusedStage<Sys, N, R, TY, ADJ, RS>;
+    if constexpr (RS > 0 && Ring::FITS) {
+        if (a.halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, RS>), grid, dim3(64 * TY), shm, stream, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ>), grid, dim3(64 * TY), shm, stream, a);
+    return hipGetLastError();
 }
 
 template <class Sys, int N, int R, bool ADJ = false>
@@ -466,8 +548,8 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     switch (tile_y) {
     case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), shm, stream, a); break;
     case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a); break;
-    case 8: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ>), grid, dim3(512), shm, stream, a); break;
-    case 16: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 16, ADJ>), grid, dim3(1024), shm, stream, a); break;
+    case 8: return launch_tile<Sys, N, R, 8, ADJ>(a, grid, shm, stream);
+    case 16: return launch_tile<Sys, N, R, 16, ADJ>(a, grid, shm, stream);
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
