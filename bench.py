@@ -95,8 +95,8 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     torch.cuda.synchronize()
     cfgs = [b.kernel]
     if tune:
-        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=za) for ty in (8, 16) for zc in (0, 64, 128)
-                for gb in (0, 512, 1024) for za in (0, 1)]
+        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=za, halo_ring=hr) for ty in (8, 16)
+                for zc in (0, 64, 128) for gb in (0, 512, 1024) for za in (0, 1) for hr in (0, 1)]
     best = None
     for cfg in cfgs:
         b.kernel = cfg

@@ -191,11 +191,13 @@ typedef struct tsp_batch_cfg {
     int32_t zchunk;      /* z planes per work item; 0 = min(nz, 32) per instance */
     int32_t grid_blocks; /* persistent grid size; 0 = one workgroup per work item */
     int32_t zalt;        /* 1: odd z chunks march downwards */
+    int32_t halo_ring;   /* 1: halo-ring tile body where its LDS fits (as tsp_kernel_cfg.halo_ring) */
 } tsp_batch_cfg;
 
 /* Every descriptor must pass tsp_create's validation, use TSP_SYS_WAVE_PHIPI
  * and a (stencil_n, ko_order) pair in {(1,0),(1,1),(2,0),(2,2),(2,3),(3,0),
- * (3,3),(4,0),(4,4)}. All registers live in one pool allocation. */
+ * (3,3),(4,0),(4,4)}, with registers under 4 GiB each. All registers live in
+ * one pool allocation. */
 tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out);
 tsp_status tsp_batch_destroy(tsp_batch *b);
 tsp_status tsp_batch_size(tsp_batch *b, int32_t *count, int64_t *cells);

@@ -34,6 +34,7 @@ def test_struct_layout_matches_c(tmp_path):
         int main(void) {
             printf("desc %zu\\n", sizeof(tsp_desc));
             printf("cfg %zu\\n", sizeof(tsp_kernel_cfg));
+            printf("bcfg %zu\\n", sizeof(tsp_batch_cfg));
     """)
     for f in fields:
         prog += f'    printf("{f} %zu\\n", offsetof(tsp_desc, {f}));\n'
@@ -46,6 +47,7 @@ def test_struct_layout_matches_c(tmp_path):
                                                        check=True).stdout.splitlines())
     assert int(out["desc"]) == C.sizeof(_lib.TspDesc)
     assert int(out["cfg"]) == C.sizeof(_lib.TspKernelCfg)
+    assert int(out["bcfg"]) == C.sizeof(_lib.TspBatchCfg)
     for f in fields:
         assert int(out[f]) == getattr(_lib.TspDesc, f).offset, f
 
@@ -123,6 +125,10 @@ def test_batch_validation():
         Batch([good.replace(stencil_n=3, ghost=4, ko_order=4)])
     with pytest.raises(_lib.TspError, match="instance 1: .*stencil_n"):
         Batch([good, good.replace(stencil_n=0)])
+    # 700^3 with two components: 5.9 GB per register, valid for one solver but
+    # beyond the batched kernels' 4 GiB buffer window
+    with pytest.raises(_lib.TspError, match="instance 1: batched instances need registers under 4 GiB"):
+        Batch([good, good.replace(n=(700, 700, 700))])
     L = _lib.lib()
     assert L.tsp_batch_create(None, 0, None, None) == _lib.INVALID_ARG
     assert L.tsp_batch_rk4_steps(None, 1) == _lib.INVALID_ARG

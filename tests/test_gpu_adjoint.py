@@ -53,6 +53,19 @@ def test_transposed_step_matches_oracle_and_dot_product(n, sn, ko, per, abl):
         assert lhs == pytest.approx(rhs, rel=1e-12)
 
 
+@pytest.mark.parametrize("n,sn,ko,per,abl", [ADJ_CASES[1], ADJ_CASES[2]])
+def test_pointer_access_transposed_step_matches_oracle(monkeypatch, n, sn, ko, per, abl):
+    """The transposed stage through the kernel that registers of 4 GiB or
+    more use (64-bit pointers), forced on small boxes."""
+    _say(f"[adjoint] pointer access n={n} stencil={sn} ko={ko}")
+    monkeypatch.setenv("TSP_POINTER_ACCESS", "1")
+    case = _case(n, sn, ko, per, abl)
+    s = Solver(case)
+    a = Adjoint(s, [[1, 1, 1]], 1, 1)
+    v = np.random.default_rng(4).standard_normal(case.shape)
+    assert np.array_equal(a.apply(v), orc.OracleSolver(case).adjoint_step(v))
+
+
 def _problem(n=(36, 30, 28), sn=2, ko=0, per=(0, 0, 0), abl=4, nsteps=9, K=4, nrec=7, seed=2):
     case = _case(n, sn, ko, per, abl, seed=seed)
     rng = np.random.default_rng(seed)

@@ -37,9 +37,10 @@ def mixed_cases():
 
 
 BATCH_CFGS = [
-    {"tile_y": 16, "zchunk": 0, "grid_blocks": 0, "zalt": 1},
-    {"tile_y": 8, "zchunk": 5, "grid_blocks": 7, "zalt": 0},
-    {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1},
+    {"tile_y": 16, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 1},
+    {"tile_y": 8, "zchunk": 5, "grid_blocks": 7, "zalt": 0, "halo_ring": 1},
+    {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1, "halo_ring": 0},
+    {"tile_y": 8, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 0},
 ]
 
 

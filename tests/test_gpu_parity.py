@@ -102,6 +102,18 @@ TILE_CFGS = [
 ]
 
 
+@pytest.mark.parametrize("system,n,stencil_n,ko,periodic,abl", [CASES[2], CASES[3], CASES[6], CASES[8]])
+def test_pointer_access_kernel_matches_oracle(monkeypatch, system, n, stencil_n, ko, periodic, abl):
+    """The stage kernel that registers of 4 GiB or more use (64-bit pointers,
+    64 x 8 tiles), forced on small boxes."""
+    _say(f"[parity] pointer access {system} n={n} stencil={stencil_n} ko={ko}")
+    monkeypatch.setenv("TSP_POINTER_ACCESS", "1")
+    case, state = small_case(system, n=n, stencil_n=stencil_n, ko_order=ko, periodic=periodic, abl=abl,
+                             gamma=0.05 if system == "nonlinear" else 0.0)
+    got, want = run_both(case, state, 3)
+    assert np.array_equal(got, want)
+
+
 def test_every_tile_shape_is_bitwise_identical():
     _say("[parity] tile shapes")
     case, state = small_case("phipi", n=(130, 27, 23), stencil_n=4, ko_order=4, periodic=(0, 1, 0), abl=4)

@@ -72,6 +72,7 @@ class TspBatchCfg(C.Structure):
         ("zchunk", C.c_int32),
         ("grid_blocks", C.c_int32),
         ("zalt", C.c_int32),
+        ("halo_ring", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

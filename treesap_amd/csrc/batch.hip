@@ -18,11 +18,12 @@
 #include <vector>
 
 namespace tsp {
-hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                 hipStream_t stream) {
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kBatchVariants>([&]<int V>() {
-        if (variant == V) e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, max_abl_T, stream);
+        if (variant == V)
+            e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, max_abl_T, halo_ring, stream);
     });
     return e;
 }
@@ -48,12 +49,12 @@ struct tsp_batch {
     int role[4] = {0, 1, 2, 3};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    tsp_batch_cfg cfg{16, 0, 0, 1};
-    tsp_batch_cfg work_cfg{-1, -1, -1, -1}; // configuration the work list was built for
+    tsp_batch_cfg cfg{16, 0, 0, 1, 1};
+    tsp_batch_cfg work_cfg{-1, -1, -1, -1, -1}; // configuration the work list was built for
     std::vector<double> time;
     int64_t steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    tsp_batch_cfg graph_cfg{-1, -1, -1, -1};
+    tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1};
 };
 
 namespace {
@@ -69,6 +70,7 @@ tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
     if (c->zchunk < 0) return fail(TSP_INVALID_ARG, "zchunk must be >= 0 (0: automatic)");
     if (c->grid_blocks < 0) return fail(TSP_INVALID_ARG, "grid_blocks must be >= 0 (0: one block per work item)");
     if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
+    if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
     return TSP_OK;
 }
 
@@ -144,7 +146,7 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
             a.zalt = b->cfg.zalt;
             a.ghost_mode = 0;
             const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->stream));
+            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
         }
     }
     std::swap(OLD, NEW);
@@ -173,6 +175,10 @@ tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, 
             return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": no batched kernel for stencil_n=" +
                                              std::to_string(descs[i].stencil_n) +
                                              " ko_order=" + std::to_string(descs[i].ko_order));
+        // the batched kernels address registers through 4 GiB buffer descriptors
+        const Layout L = layout_of(descs[i]);
+        if ((L.elems + (int64_t)L.G * (1 + L.sj + L.sk)) * (int64_t)sizeof(double) >= (int64_t(1) << 32))
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": batched instances need registers under 4 GiB");
     }
     {
         int maxT = 0;

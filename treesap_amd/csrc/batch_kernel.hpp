@@ -53,9 +53,9 @@ inline int batch_variant(int n, int r) {
 }
 
 // One (stencil width, KO order) group: a persistent grid strides over the
-// group's work items.
-template <class Sys, int N, int R, int TY> struct BatchedStage {
-    using FS = FusedStage<Sys, N, R, TY>;
+// group's work items. RS > 0: the halo-ring tile body.
+template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
+    using FS = FusedStage<Sys, N, R, TY, false, RS>;
 
     __device__ __forceinline__ static void run(const BatchInstance *__restrict__ inst,
                                                const BatchWork *__restrict__ work, const BatchArgs &b, double *lds,
@@ -87,6 +87,8 @@ template <class Sys, int N, int R, int TY> struct BatchedStage {
             a.tiles_x = a.tiles_y = a.nzc = 0;
             a.xcd_remap = 0;
             a.zalt = b.zalt;
+            a.halo_ring = RS > 0;
+            a.ptr_access = 0;
             // integrators/SingleLevelRK4Integrator.hpp:71-74,119-121 (the same
             // products the host forms for a single box)
             const int s = b.stage;
@@ -117,35 +119,45 @@ template <class Sys, int N, int R, int TY> struct BatchedStage {
 // The instance table and work list are separate restrict-qualified arguments:
 // nothing the kernel stores can alias them, so their uniform loads can be
 // scalar loads.
-template <class Sys, int N, int R, int TY>
+template <class Sys, int N, int R, int TY, int RS = 0>
 __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInstance *__restrict__ inst,
                                                                 const BatchWork *__restrict__ work,
                                                                 const BatchArgs b) {
-    __shared__ double lds[FusedStage<Sys, N, R, TY>::LDS_DOUBLES];
+    __shared__ double lds[FusedStage<Sys, N, R, TY, false, RS>::LDS_TOTAL];
     extern __shared__ double sig_lds[];
-    BatchedStage<Sys, N, R, TY>::run(inst, work, b, lds, sig_lds);
+    BatchedStage<Sys, N, R, TY, RS>::run(inst, work, b, lds, sig_lds);
 }
 
-// `max_abl_T`: the thickest absorbing layer in the group (sizes the LDS rate table)
-template <class Sys, int N, int R> hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_T,
-                                                            hipStream_t stream) {
-    const size_t shm = sigma_lds_bytes(max_abl_T);
-    switch (tile_y) {
-    case 8:
-        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 8>), dim3(grid), dim3(512), shm, stream, b.inst, b.work,
-                           b);
-        break;
-    case 16:
-        hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, 16>), dim3(grid), dim3(1024), shm, stream, b.inst,
-                           b.work, b);
-        break;
-    default: return hipErrorInvalidValue;
+// one tile height: the halo-ring kernel when asked for and its LDS fits
+template <class Sys, int N, int R, int TY>
+hipError_t launch_batched_tile(const BatchArgs &b, int grid, size_t shm, int halo_ring, hipStream_t stream) {
+    constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY>::H>();
+    using Ring = FusedStage<Sys, N, R, TY, false, RS>;
+    if constexpr (RS > 0 && Ring::FITS) {
+        if (halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
+            hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS>), dim3(grid), dim3(64 * TY), shm, stream,
+                               b.inst, b.work, b);
+            return hipGetLastError();
+        }
     }
+    hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY>), dim3(grid), dim3(64 * TY), shm, stream, b.inst, b.work,
+                       b);
     return hipGetLastError();
 }
 
+// `max_abl_T`: the thickest absorbing layer in the group (sizes the LDS rate table)
+template <class Sys, int N, int R>
+hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring, hipStream_t stream) {
+    const size_t shm = sigma_lds_bytes(max_abl_T);
+    switch (tile_y) {
+    case 8: return launch_batched_tile<Sys, N, R, 8>(b, grid, shm, halo_ring, stream);
+    case 16: return launch_batched_tile<Sys, N, R, 16>(b, grid, shm, halo_ring, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // launch one variant group of the (phi, Pi) batch
-hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                 hipStream_t stream);
 
 } // namespace tsp

@@ -40,6 +40,9 @@ struct tsp_solver {
     int role[4] = {0, 1, 2, 3}; // role -> register index
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // TSP_POINTER_ACCESS set at creation: run the pointer-access stage kernel
+    // that registers of 4 GiB or more use, on any size (for testing it)
+    bool ptr_access = false;
     double *sigma_tab = nullptr;
     tsp::LaunchFn launch = nullptr;
     tsp_kernel_cfg cfg{0, 8, 64, 1, 1, 1};
@@ -136,6 +139,7 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     a.xcd_remap = s->cfg.xcd_remap;
     a.zalt = s->cfg.zalt;
     a.halo_ring = s->cfg.halo_ring;
+    a.ptr_access = s->ptr_access ? 1 : 0;
     a.b = b;
     a.a_next = a_next;
     for (int i = 0; i < 3; ++i) {
@@ -257,6 +261,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     s->reg_elems = s->sc * s->nc;
     s->launch = lookup_launcher(d->system, d->stencil_n, d->ko_order);
     s->cfg = default_cfg(*d);
+    s->ptr_access = std::getenv("TSP_POINTER_ACCESS") != nullptr;
     // Geometry::CellSize = (prob_hi - prob_lo) / n_cells; grid factors as the stencils form them
     for (int a = 0; a < 3; ++a) {
         const double dx = (d->prob_hi[a] - d->prob_lo[a]) / (double)d->n[a];

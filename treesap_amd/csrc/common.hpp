@@ -120,7 +120,8 @@ struct StageArgs {
     int32_t tiles_x, tiles_y, nzc;
     int32_t xcd_remap;
     int32_t zalt;            // alternate the z-march direction of consecutive chunks
-    int32_t halo_ring;       // 64x16 tiles: load halos with the queue head, park them in LDS
+    int32_t halo_ring;       // 64x8/64x16 tiles: load halos with the queue head, park them in LDS
+    int32_t ptr_access;      // 1: 64-bit pointer addressing (registers of 4 GiB or more)
     double b, a_next;        // dt*b_s and dt*a_{s+1} (integrators/SingleLevelRK4Integrator.hpp:119-121)
     double c_init_argn;      // adjoint only: coefficient of lambda' in the next stage argument
     double src_scale;        // adjoint only: factor applied to every source value on load

@@ -28,6 +28,8 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace tsp {
 
 // ---------------------------------------------------------------------------
@@ -180,12 +182,78 @@ __device__ __forceinline__ int map_axis(int i, int n, int wrap, int G) {
     return i;
 }
 
+// Register access. Elements are addressed as (column, plane, component):
+// column = i + j*sj (i, j >= -G), plane index k >= -G.
+//
+// BufAccess: one buffer descriptor per register, based at element (-G,-G,-G)
+// of component 0; the lane's column becomes a loop-invariant byte offset in a
+// VGPR and the plane and component a scalar byte offset, so an access costs no
+// vector address arithmetic. Needs every register to span less than 4 GiB.
+struct BufAccess {
+    static constexpr int kNT = 2; // nontemporal cache policy of a buffer access
+    __amdgpu_buffer_rsrc_t rs, ri, ra, rn;
+    uint32_t cb, skb, scb;
+    int G;
+    __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rsrc(const double *origin, const StageArgs &a) {
+        const int64_t shift = (int64_t)a.G * (1 + (int64_t)a.sj + (int64_t)a.sk);
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(origin - shift), 0, -1, 0x00020000);
+    }
+    __device__ __forceinline__ explicit BufAccess(const StageArgs &a)
+        : rs(rsrc(a.src, a)), ri(rsrc(a.init, a)), ra(rsrc(a.acc, a)), rn(rsrc(a.argn ? a.argn : a.acc, a)),
+          cb((uint32_t)(a.G + a.G * a.sj)), skb((uint32_t)a.sk * 8u), scb((uint32_t)a.sc * 8u), G(a.G) {}
+    __device__ __forceinline__ uint32_t col(int c) const { return ((uint32_t)c + cb) * 8u; }
+    // the plane offset is uniform; readfirstlane pins it to an SGPR (a scalar
+    // offset the compiler computed in VGPRs would need a waterfall loop)
+    __device__ __forceinline__ uint32_t plane(int k) const {
+        return __builtin_amdgcn_readfirstlane((uint32_t)(k + G) * skb);
+    }
+    template <int AUX> __device__ __forceinline__ static double ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, AUX));
+    }
+    __device__ __forceinline__ static void st(double v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, vo, so, kNT);
+    }
+    template <int C> __device__ __forceinline__ double src(uint32_t c, uint32_t p) const { return ld<0>(rs, c, p + C * scb); }
+    template <int C> __device__ __forceinline__ double init(uint32_t c, uint32_t p) const { return ld<kNT>(ri, c, p + C * scb); }
+    template <int C> __device__ __forceinline__ double acc(uint32_t c, uint32_t p) const { return ld<kNT>(ra, c, p + C * scb); }
+    template <int C> __device__ __forceinline__ void put_acc(double v, uint32_t c, uint32_t p) const { st(v, ra, c, p + C * scb); }
+    template <int C> __device__ __forceinline__ void put_argn(double v, uint32_t c, uint32_t p) const { st(v, rn, c, p + C * scb); }
+};
+
+// PtrAccess: plain 64-bit pointers, for registers of 4 GiB or more.
+struct PtrAccess {
+    const StageArgs &a;
+    __device__ __forceinline__ explicit PtrAccess(const StageArgs &a_) : a(a_) {}
+    __device__ __forceinline__ int col(int c) const { return c; }
+    __device__ __forceinline__ int64_t plane(int k) const { return (int64_t)k * a.sk; }
+    template <int C> __device__ __forceinline__ double src(int c, int64_t p) const { return (a.src + C * (int64_t)a.sc + p)[c]; }
+    template <int C> __device__ __forceinline__ double init(int c, int64_t p) const {
+        return __builtin_nontemporal_load(&(a.init + C * (int64_t)a.sc + p)[c]);
+    }
+    template <int C> __device__ __forceinline__ double acc(int c, int64_t p) const {
+        return __builtin_nontemporal_load(&(a.acc + C * (int64_t)a.sc + p)[c]);
+    }
+    template <int C> __device__ __forceinline__ void put_acc(double v, int c, int64_t p) const {
+        __builtin_nontemporal_store(v, &(a.acc + C * (int64_t)a.sc + p)[c]);
+    }
+    template <int C> __device__ __forceinline__ void put_argn(double v, int c, int64_t p) const {
+        __builtin_nontemporal_store(v, &(a.argn + C * (int64_t)a.sc + p)[c]);
+    }
+};
+
+// Whether every register of a launch spans less than 4 GiB (buffer access).
+inline bool regs_fit_buffer(const StageArgs &a, int ncomp) {
+    const int64_t span = (int64_t)a.G * (1 + (int64_t)a.sj + (int64_t)a.sk) + (int64_t)ncomp * a.sc;
+    return span * (int64_t)sizeof(double) < (int64_t(1) << 32);
+}
+
 // RS > 0 (halo ring): the x/y halo of a plane is loaded RS+1 steps ahead of
 // its use, about when the neighbouring tiles load those cells as their own
 // z-queue heads (H+2 steps ahead) and L2 still holds them, and parked in an
 // LDS ring of RS slots until the plane is computed. Each thread stores and
 // reloads only its own halo cells, so the ring needs no barrier.
-template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct FusedStage {
+template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BUF = true> struct FusedStage {
     static constexpr int TX = 64;
     static constexpr int H = N > R ? N : R;
     static constexpr int NC = Sys::NC;
@@ -228,11 +296,14 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
         }
     };
 
-    // Source value of component C at column offset `col` of the plane whose
-    // offset is `zoff` (the adjoint's first reverse stage reads b_3 * lambda'
-    // without materialising it).
-    template <int C> __device__ __forceinline__ static double src_at(const StageArgs &a, int col, int zoff) {
-        const double v = (a.src + C * a.sc + zoff)[col];
+    using Access = std::conditional_t<BUF, BufAccess, PtrAccess>;
+
+    // Source value of component C at column handle `col` of plane handle `pl`
+    // (the adjoint's first reverse stage reads b_3 * lambda' without
+    // materialising it).
+    template <int C, class Col, class Pl>
+    __device__ __forceinline__ static double src_at(const StageArgs &a, const Access &m, Col col, Pl pl) {
+        const double v = m.template src<C>(col, pl);
         if constexpr (ADJ) return a.src_scale * v;
         else return v;
     }
@@ -283,13 +354,16 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
         const int wrapx = a.ghost_mode == 0 && a.periodic[0];
         const int wrapy = a.ghost_mode == 0 && a.periodic[1];
         const int wrapz = a.ghost_mode == 0 && a.periodic[2];
-        auto zoff_of = [&](int k) { return map_axis(k, a.nz, wrapz, a.G) * a.sk; };
+        const Access mem(a);
+        auto zoff_of = [&](int k) { return mem.plane(map_axis(k, a.nz, wrapz, a.G)); };
         auto plane = [&](int t) { return DOWN ? z1 - 1 - t : z0 + t; };          // z of step t
         auto qplane = [&](int t) { return DOWN ? z1 - 1 - t - H : z0 + t + H; }; // queue head of step t
 
         // column offsets of this lane's own column and of its halo cells
         const int qcol = map_axis(gi, a.nx, wrapx, a.G) + map_axis(gj, a.ny, wrapy, a.G) * a.sj;
         const int ccol = active ? gi + gj * a.sj : qcol; // centre column (clamped for lanes outside the box)
+        const auto qh = mem.col(qcol), ch = mem.col(ccol);
+        decltype(mem.col(0)) hh[HPT];
         int hcol[HPT], hlds[HPT];
         bool hok[HPT];
 #pragma unroll
@@ -308,6 +382,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
             }
             hok[m] = h < NHALO;
             hcol[m] = hok[m] ? map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj : qcol;
+            hh[m] = mem.col(hcol[m]);
             hlds[m] = (hy + H) * LW + hx + H;
         }
 
@@ -323,19 +398,19 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
         double q[NS][Q], qn[NS], hv[NS][HPT];
 
         auto load_halo = [&](int t) {
-            const int zh = zoff_of(plane(t));
+            const auto zh = zoff_of(plane(t));
             static_for<0, NC>([&]<int C>() {
                 if constexpr (Staging<Sys, R>::staged(C)) {
                     constexpr int sl = Staging<Sys, R>::slot(C);
 #pragma unroll
-                    for (int m = 0; m < HPT; ++m) hv[sl][m] = src_at<C>(a, hcol[m], zh);
+                    for (int m = 0; m < HPT; ++m) hv[sl][m] = src_at<C>(a, mem, hh[m], zh);
                 }
             });
         };
         auto load_queue = [&](int t) {
-            const int zq = zoff_of(qplane(t));
+            const auto zq = zoff_of(qplane(t));
             static_for<0, NC>([&]<int C>() {
-                if constexpr (Staging<Sys, R>::staged(C)) qn[Staging<Sys, R>::slot(C)] = src_at<C>(a, qcol, zq);
+                if constexpr (Staging<Sys, R>::staged(C)) qn[Staging<Sys, R>::slot(C)] = src_at<C>(a, mem, qh, zq);
             });
         };
 
@@ -350,7 +425,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
                 constexpr int sl = Staging<Sys, R>::slot(C);
 #pragma unroll
                 for (int m = 0; m < Q; ++m)
-                    q[sl][m] = src_at<C>(a, qcol, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
+                    q[sl][m] = src_at<C>(a, mem, qh, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
             }
         });
         if constexpr (RS > 0) {
@@ -372,13 +447,13 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
 
         for (int t = 0; t < cnt; ++t) {
             const int z = plane(t);
-            const int off = ccol + z * a.sk;
+            const auto zp = mem.plane(z);
             // centre values of this plane: init, acc and the source of unstaged components
             double iv[NC], av[NC], cv[NC];
             static_for<0, NC>([&]<int C>() {
-                if constexpr (!Staging<Sys, R>::staged(C)) cv[C] = src_at<C>(a, off, 0);
-                if (read_init) iv[C] = __builtin_nontemporal_load(&a.init[off + C * a.sc]);
-                if (read_acc) av[C] = __builtin_nontemporal_load(&a.acc[off + C * a.sc]);
+                if constexpr (!Staging<Sys, R>::staged(C)) cv[C] = src_at<C>(a, mem, ch, zp);
+                if (read_init) iv[C] = mem.template init<C>(ch, zp);
+                if (read_acc) av[C] = mem.template acc<C>(ch, zp);
             });
             // this plane into one half of the double-buffered LDS (the other
             // half may still be read by waves finishing the previous plane)
@@ -442,8 +517,8 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
                         else accv = av[C] + a.b * k;
                         if (write_argn) argv = initc + a.a_next * k;
                     }
-                    __builtin_nontemporal_store(accv, &a.acc[off + C * a.sc]);
-                    if (write_argn) __builtin_nontemporal_store(argv, &a.argn[off + C * a.sc]);
+                    mem.template put_acc<C>(accv, ch, zp);
+                    if (write_argn) mem.template put_argn<C>(argv, ch, zp);
                 });
             }
 
@@ -495,9 +570,9 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0> struct 
 // dynamic LDS bytes for the ABL rate table of thickness T
 inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
-template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0>
+template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true>
 __global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
-    FusedStage<Sys, N, R, TY, ADJ, RS>::run(a);
+    FusedStage<Sys, N, R, TY, ADJ, RS, BUF>::run(a);
 }
 
 // Ring slots of the halo-ring variant by tile height: H+1 for 64x16 tiles (one
@@ -538,6 +613,9 @@ hipError_t launch_tile(const StageArgs &a, dim3 grid, size_t shm, hipStream_t st
 template <class Sys, int N, int R, bool ADJ = false>
 hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     StageArgs a = a0;
+    // registers of 4 GiB or more run one pointer-access kernel (64 x 8 tiles)
+    const bool buf = !a.ptr_access && regs_fit_buffer(a, Sys::NC);
+    if (!buf) tile_y = 8;
     a.tiles_x = (a.nx + 63) / 64;
     a.tiles_y = (a.ny + tile_y - 1) / tile_y;
     a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
@@ -545,6 +623,10 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     if (nblocks <= 0 || nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)nblocks);
     const size_t shm = sigma_lds_bytes(a.abl_T);
+    if (!buf) {
+        hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ, 0, false>), grid, dim3(512), shm, stream, a);
+        return hipGetLastError();
+    }
     switch (tile_y) {
     case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), shm, stream, a); break;
     case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a); break;
