@@ -192,6 +192,7 @@ typedef struct tsp_batch_cfg {
     int32_t grid_blocks; /* persistent grid size; 0 = one workgroup per work item */
     int32_t zalt;        /* 1: odd z chunks march downwards */
     int32_t halo_ring;   /* 1: halo-ring tile body where its LDS fits (as tsp_kernel_cfg.halo_ring) */
+    int32_t narrow_tiles; /* 1: an x remainder of <= 32 cells runs as 32 x 2*tile_y tiles (no idle lanes) */
 } tsp_batch_cfg;
 
 /* Every descriptor must pass tsp_create's validation, use TSP_SYS_WAVE_PHIPI

@@ -27,6 +27,7 @@ def mixed_cases():
         ((32, 32, 32), 4, 0, (1, 1, 1), 0),
         ((48, 16, 8), 2, 0, (0, 0, 0), 2),
         ((64, 20, 30), 3, 0, (1, 1, 1), 0),
+        ((96, 40, 12), 2, 2, (1, 1, 1), 0),
     ]
     out = []
     for k, (n, sn, ko, per, abl) in enumerate(specs):
@@ -37,10 +38,11 @@ def mixed_cases():
 
 
 BATCH_CFGS = [
-    {"tile_y": 16, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 1},
-    {"tile_y": 8, "zchunk": 5, "grid_blocks": 7, "zalt": 0, "halo_ring": 1},
-    {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1, "halo_ring": 0},
-    {"tile_y": 8, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 0},
+    {"tile_y": 16, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 1, "narrow_tiles": 1},
+    {"tile_y": 8, "zchunk": 5, "grid_blocks": 7, "zalt": 0, "halo_ring": 1, "narrow_tiles": 1},
+    {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1, "halo_ring": 0, "narrow_tiles": 1},
+    {"tile_y": 8, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 0, "narrow_tiles": 0},
+    {"tile_y": 16, "zchunk": 3, "grid_blocks": 0, "zalt": 1, "halo_ring": 1, "narrow_tiles": 0},
 ]
 
 

@@ -73,6 +73,7 @@ class TspBatchCfg(C.Structure):
         ("grid_blocks", C.c_int32),
         ("zalt", C.c_int32),
         ("halo_ring", C.c_int32),
+        ("narrow_tiles", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

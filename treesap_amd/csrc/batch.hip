@@ -49,12 +49,12 @@ struct tsp_batch {
     int role[4] = {0, 1, 2, 3};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    tsp_batch_cfg cfg{16, 0, 0, 1, 1};
-    tsp_batch_cfg work_cfg{-1, -1, -1, -1, -1}; // configuration the work list was built for
+    tsp_batch_cfg cfg{16, 0, 0, 1, 1, 1};
+    tsp_batch_cfg work_cfg{-1, -1, -1, -1, -1, -1}; // configuration the work list was built for
     std::vector<double> time;
     int64_t steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1};
+    tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1, -1};
 };
 
 namespace {
@@ -71,13 +71,15 @@ tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
     if (c->grid_blocks < 0) return fail(TSP_INVALID_ARG, "grid_blocks must be >= 0 (0: one block per work item)");
     if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
     if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
+    if (c->narrow_tiles != 0 && c->narrow_tiles != 1) return fail(TSP_INVALID_ARG, "narrow_tiles must be 0 or 1");
     return TSP_OK;
 }
 
 int auto_zchunk(int nz) { return std::min(nz, 32); }
 
 // (Re)build the device work list: instances in decreasing cell count, each
-// split into 64 x tile_y column tiles and z chunks.
+// split into 64 x tile_y column tiles and z chunks; with narrow_tiles, an x
+// remainder of at most 32 cells becomes a strip of 32 x 2*tile_y tiles.
 tsp_status build_work(tsp_batch *b) {
     if (b->d_work && std::memcmp(&b->work_cfg, &b->cfg, sizeof(tsp_batch_cfg)) == 0) return TSP_OK;
     std::vector<int> order(b->count);
@@ -97,9 +99,14 @@ tsp_status build_work(tsp_batch *b) {
             const int zc = b->cfg.zchunk > 0 ? std::min(b->cfg.zchunk, d.n[2]) : auto_zchunk(d.n[2]);
             b->h_inst[i].zchunk = zc;
             const int nzc = (d.n[2] + zc - 1) / zc;
-            for (int c = 0; c < nzc; ++c)
+            // x origin of the narrow strip (nx if there is none)
+            const int xn = b->cfg.narrow_tiles && d.n[0] % 64 != 0 && d.n[0] % 64 <= 32 ? d.n[0] / 64 * 64 : d.n[0];
+            for (int c = 0; c < nzc; ++c) {
                 for (int y0 = 0; y0 < d.n[1]; y0 += ty)
-                    for (int x0 = 0; x0 < d.n[0]; x0 += 64) work.push_back(tsp::BatchWork{i, x0, y0, c});
+                    for (int x0 = 0; x0 < xn; x0 += 64) work.push_back(tsp::BatchWork{i, x0, y0, c, 0});
+                if (xn < d.n[0])
+                    for (int y0 = 0; y0 < d.n[1]; y0 += 2 * ty) work.push_back(tsp::BatchWork{i, xn, y0, c, 1});
+            }
         }
         b->group_n[v] = (int)work.size() - b->group_off[v];
     }

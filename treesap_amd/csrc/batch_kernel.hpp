@@ -29,6 +29,7 @@ struct BatchInstance {
 
 struct BatchWork {
     int32_t inst, x0, y0, zc;
+    int32_t narrow; // 1: a 32 x 2*TY tile (the 32-wide x remainder of a box)
 };
 
 struct BatchArgs {
@@ -53,9 +54,13 @@ inline int batch_variant(int n, int r) {
 }
 
 // One (stencil width, KO order) group: a persistent grid strides over the
-// group's work items. RS > 0: the halo-ring tile body.
+// group's work items. RS > 0: the halo-ring tile body. Items are 64 x TY
+// tiles or, flagged narrow, 32 x 2*TY tiles (same thread count).
 template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
     using FS = FusedStage<Sys, N, R, TY, false, RS>;
+    using NFS = FusedStage<Sys, N, R, 2 * TY, false, RS, true, 32>;
+    static constexpr int LDS_TOTAL = FS::LDS_TOTAL > NFS::LDS_TOTAL ? FS::LDS_TOTAL : NFS::LDS_TOTAL;
+    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 120 * 1024;
 
     __device__ __forceinline__ static void run(const BatchInstance *__restrict__ inst,
                                                const BatchWork *__restrict__ work, const BatchArgs &b, double *lds,
@@ -110,8 +115,14 @@ template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
             const int z1 = min(z0 + I.zchunk, I.nz);
             __syncthreads(); // the previous item's waves may still read LDS
             const double *sig = FS::stage_sigma(I.sigma_tab, I.abl_T, sig_lds);
-            if (b.zalt && (wk.zc & 1)) FS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds, sig);
-            else FS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+            const bool down = b.zalt && (wk.zc & 1);
+            if (wk.narrow) {
+                if (down) NFS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+                else NFS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+            } else {
+                if (down) FS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+                else FS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds, sig);
+            }
         }
     }
 };
@@ -123,7 +134,7 @@ template <class Sys, int N, int R, int TY, int RS = 0>
 __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInstance *__restrict__ inst,
                                                                 const BatchWork *__restrict__ work,
                                                                 const BatchArgs b) {
-    __shared__ double lds[FusedStage<Sys, N, R, TY, false, RS>::LDS_TOTAL];
+    __shared__ double lds[BatchedStage<Sys, N, R, TY, RS>::LDS_TOTAL];
     extern __shared__ double sig_lds[];
     BatchedStage<Sys, N, R, TY, RS>::run(inst, work, b, lds, sig_lds);
 }
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInsta
 template <class Sys, int N, int R, int TY>
 hipError_t launch_batched_tile(const BatchArgs &b, int grid, size_t shm, int halo_ring, hipStream_t stream) {
     constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY>::H>();
-    using Ring = FusedStage<Sys, N, R, TY, false, RS>;
+    using Ring = BatchedStage<Sys, N, R, TY, RS>;
     if constexpr (RS > 0 && Ring::FITS) {
         if (halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
             hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS>), dim3(grid), dim3(64 * TY), shm, stream,

@@ -253,8 +253,13 @@ inline bool regs_fit_buffer(const StageArgs &a, int ncomp) {
 // z-queue heads (H+2 steps ahead) and L2 still holds them, and parked in an
 // LDS ring of RS slots until the plane is computed. Each thread stores and
 // reloads only its own halo cells, so the ring needs no barrier.
-template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BUF = true> struct FusedStage {
-    static constexpr int TX = 64;
+//
+// TXW: tile width, 64 (one wave per x row) or 32 (each wave covers two rows;
+// for boxes whose x extent leaves a 32-wide remainder).
+template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BUF = true, int TXW = 64>
+struct FusedStage {
+    static_assert(TXW == 64 || TXW == 32, "tile width 64 or 32");
+    static constexpr int TX = TXW;
     static constexpr int H = N > R ? N : R;
     static constexpr int NC = Sys::NC;
     static constexpr int NS = Staging<Sys, R>::count();
@@ -308,7 +313,7 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BU
         else return v;
     }
 
-    // One 64 x TY column tile over z planes [z0, z1). `lds_raw` holds
+    // One TX x TY column tile over z planes [z0, z1). `lds_raw` holds
     // LDS_DOUBLES doubles of LDS; `sig` is the ABL rate table by squared
     // distance, resident in LDS (null without an ABL). Every thread of the
     // block must call this.
@@ -330,8 +335,8 @@ template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BU
     __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
                                                     double *lds_raw, const double *sig) {
         const int tid = threadIdx.x;
-        const int lane = tid & 63;
-        const int row = tid >> 6;
+        const int lane = tid & (TX - 1);
+        const int row = tid / TX;
         const int gi = x0 + lane, gj = y0 + row;
         const bool active = gi < a.nx && gj < a.ny;
         const int cnt = z1 - z0;
