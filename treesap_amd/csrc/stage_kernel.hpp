@@ -324,8 +324,9 @@ struct FusedStage {
     //   nu = L^T mu;  acc = (first ? lambda' : acc) + nu;  argn = b_{j-1}*lambda' + a_j*nu
     //
     // Memory pipeline. While plane t is computed, the x/y halo of plane t+1
-    // and the z-queue head of plane t+2 are in flight; the init/acc values of
-    // plane t are issued at the top of its step and consumed at its end. The
+    // (t+RS+1 with the ring) and the z-queue head of plane t+H+1 are in
+    // flight; the init/acc values of plane t are issued at the top of its step
+    // and consumed at its end. The
     // loads that carry values into the next step are unconditional (their
     // indices are clamped into the allocation; values beyond the chunk are
     // never used), so each one lands in a fixed register and no copy forces
@@ -423,16 +424,20 @@ struct FusedStage {
         double *ring = lds_raw + LDS_DOUBLES;
         auto ring_at = [&](int r, int s, int m) -> double & { return ring[(r * NS + s) * NHALO + tid + m * NT]; };
 
-        // prologue: queue and halo of step 0, queue head of step 1; with the
-        // ring, the halos of steps 0..RS-1 parked and that of step RS in flight
+        // prologue: the queue as it stands before step 0 advances it (planes
+        // -H-1..H-1 of the chunk; the first is shifted out unread) and the
+        // queue head of step 0 in flight; the halo of step 0 in flight, or with
+        // the ring, the halos of steps 0..RS-1 parked and that of step RS in flight
         static_for<0, NC>([&]<int C>() {
             if constexpr (Staging<Sys, R>::staged(C)) {
                 constexpr int sl = Staging<Sys, R>::slot(C);
+                q[sl][0] = 0.0;
 #pragma unroll
-                for (int m = 0; m < Q; ++m)
-                    q[sl][m] = src_at<C>(a, mem, qh, zoff_of(DOWN ? z1 - 1 + H - m : z0 - H + m));
+                for (int m = 1; m < Q; ++m)
+                    q[sl][m] = src_at<C>(a, mem, qh, zoff_of(DOWN ? z1 + H - m : z0 - H - 1 + m));
             }
         });
+        load_queue(0);
         if constexpr (RS > 0) {
 #pragma unroll
             for (int p = 0; p < RS; ++p) {
@@ -447,12 +452,22 @@ struct FusedStage {
         } else {
             load_halo(0);
         }
-        load_queue(min(1, tlast));
         int rslot = 0; // ring slot of step t: t % RS
 
         for (int t = 0; t < cnt; ++t) {
             const int z = plane(t);
             const auto zp = mem.plane(z);
+            // advance the queue to planes z-H..z+H with the head loaded during
+            // step t-1. The shift comes first in the step so that the old head
+            // is dead before the next head is loaded into the same registers
+            // (a head loaded before the shift would need a copy that waits for
+            // the load just issued).
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+#pragma unroll
+                for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
+                q[s][Q - 1] = qn[s];
+            }
             // centre values of this plane: init, acc and the source of unstaged components
             double iv[NC], av[NC], cv[NC];
             static_for<0, NC>([&]<int C>() {
@@ -460,6 +475,7 @@ struct FusedStage {
                 if (read_init) iv[C] = mem.template init<C>(ch, zp);
                 if (read_acc) av[C] = mem.template acc<C>(ch, zp);
             });
+            load_queue(min(t + 1, tlast));
             // this plane into one half of the double-buffered LDS (the other
             // half may still be read by waves finishing the previous plane)
             double *buf = lds_raw + (t & 1) * LDS_PLANE;
@@ -527,14 +543,6 @@ struct FusedStage {
                 });
             }
 
-            // advance the queue (its head for step t+1 was loaded during step t-1)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-#pragma unroll
-                for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
-                q[s][Q - 1] = qn[s];
-            }
-            load_queue(min(t + 2, tlast));
         }
     }
 
