@@ -231,13 +231,16 @@ template <int N, int R> struct TensorTile {
         for (int c = 0; c < 6; ++c) {
             __syncthreads(); // the previous component's reads of u and g are done
             stage_region();
-            if (c < 5 || R > 0) load_region(c + 1); // next h component, or the first K component for KO
+            // centre values first, then the prefetch: the memory counter
+            // completes in order, so the update's wait for the centre values
+            // then leaves the next region's loads in flight
             double kc = 0.0, initv = 0.0, accv = 0.0;
             if (active) {
                 kc = a.src[off + (long long)(6 + c) * a.sc];
                 if (read_init) initv = a.init[off + (long long)c * a.sc];
                 if (!is_first) accv = a.acc[off + (long long)c * a.sc];
             }
+            if (c < 5 || R > 0) load_region(c + 1); // next h component, or the first K component for KO
             __syncthreads();
 
             double dv[3][3];
@@ -293,19 +296,25 @@ template <int N, int R> struct TensorTile {
 #pragma unroll 1
         for (int c = 6; c < 12; ++c) {
             double ko_term = 0.0, cenc = 0.0, initv = 0.0, accv = 0.0;
+            auto load_centre = [&]() {
+                if (active) {
+                    if (read_init) initv = a.init[off + (long long)c * a.sc];
+                    if (!is_first) accv = a.acc[off + (long long)c * a.sc];
+                }
+            };
             if constexpr (R > 0) {
                 __syncthreads();
                 stage_region();
-                if (c < 11) load_region(c + 1);
+                load_centre(); // before the prefetch (see the h loop)
+                // unconditional (the last component rereads its own, cached
+                // region), so the wait counting stays exact through the loop
+                load_region(min(c + 1, 11));
                 __syncthreads();
                 ko_term = ko_sum();
                 cenc = u[own];
             } else {
                 if (active) cenc = a.src[off + (long long)c * a.sc];
-            }
-            if (active) {
-                if (read_init) initv = a.init[off + (long long)c * a.sc];
-                if (!is_first) accv = a.acc[off + (long long)c * a.sc];
+                load_centre();
             }
             const int row = c >= 9 ? 2 : c >= 7 ? 1 : 0;
             const int col = (c - 6) - (row * (row + 1)) / 2;
