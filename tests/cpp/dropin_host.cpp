@@ -6,6 +6,7 @@
 //
 //   dropin_host selftest
 //   dropin_host eval <case> <nx> <ny> <nz> <px> <py> <pz> <bc> <in.bin> <out.bin>
+//   dropin_host expr <file>   (lines "x y z<TAB>expression"; prints each value)
 // The eval input is the valid state a[((c*nz+k)*ny+j)*nx+i] (doubles); ghost
 // cells are filled like the library's FillPatch (periodic images, then the
 // boundary functor outside non-periodic faces, with Dirichlet value 0).
@@ -336,6 +337,24 @@ int main(int argc, char **argv) {
         fout.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)(out.size() * sizeof(double)));
         return fout ? 0 : 1;
     }
-    std::fprintf(stderr, "usage: %s selftest [dir] | eval <case> nx ny nz px py pz bc in out\n", argv[0]);
+    if (mode == "expr" && argc == 3) {
+        // one "x y z<TAB>expression" per line of the file; prints the value of each
+        std::ifstream fin(argv[2]);
+        std::string line;
+        while (std::getline(fin, line)) {
+            const auto tab = line.find('\t');
+            if (tab == std::string::npos) return 2;
+            double x = 0, y = 0, z = 0;
+            if (std::sscanf(line.substr(0, tab).c_str(), "%lf %lf %lf", &x, &y, &z) != 3) return 2;
+            try {
+                const FunctionInit<1> f({line.substr(tab + 1)});
+                std::printf("%.17g\n", f.operator()<0>(x, y, z));
+            } catch (const std::exception &e) {
+                std::printf("error %s\n", e.what());
+            }
+        }
+        return 0;
+    }
+    std::fprintf(stderr, "usage: %s selftest [dir] | eval <case> nx ny nz px py pz bc in out | expr <file>\n", argv[0]);
     return 2;
 }

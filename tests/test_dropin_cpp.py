@@ -7,6 +7,7 @@ build entry point); a missing binary is built on first use.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import subprocess
 import sys
 import zlib
@@ -35,6 +36,64 @@ def _run(args, **kw):
 def test_host_selftest(tmp_path):
     res = _run([cpp_build.binary("dropin_host"), "selftest", tmp_path])
     assert "passed" in res.stdout
+
+
+# FunctionInit expressions against Python's evaluation of the same arithmetic.
+# The reference hands these strings to AMReX's parser, which is not in the
+# reference tree; what is checked is the grammar's mathematical meaning:
+# precedence, left-associative + - * /, right-associative power that binds
+# tighter than unary minus ("-x^2" is -(x^2), "2^-1" is 0.5), the functions
+# and pi. Python's float operators and math functions call the same libm, so
+# the values agree to the bit.
+EXPR_CASES = [
+    ("x + y * z", lambda x, y, z: x + y * z),
+    ("x - y - z", lambda x, y, z: x - y - z),
+    ("x / y / z", lambda x, y, z: x / y / z),
+    ("x * y / z * x", lambda x, y, z: x * y / z * x),
+    ("(x + y) * (z - x)", lambda x, y, z: (x + y) * (z - x)),
+    ("-x^2", lambda x, y, z: -(x ** 2)),
+    ("-x**2 + y", lambda x, y, z: -(x ** 2) + y),
+    ("2^3^2", lambda x, y, z: 2.0 ** (3.0 ** 2.0)),
+    ("2^-1 + x^-2", lambda x, y, z: 2.0 ** -1.0 + x ** -2.0),
+    ("--x", lambda x, y, z: -(-x)),
+    ("+x - -y", lambda x, y, z: x - (-y)),
+    ("x*-y", lambda x, y, z: x * (-y)),
+    ("1e-3 * x + 2.5E2", lambda x, y, z: 1e-3 * x + 2.5e2),
+    ("exp(-(x^2 + y^2 + z^2)/0.1^2)", lambda x, y, z: math.exp(-(x ** 2 + y ** 2 + z ** 2) / 0.1 ** 2)),
+    ("sqrt(x*x + y*y)", lambda x, y, z: math.sqrt(x * x + y * y)),
+    ("sin(pi*x) * cos(2*pi*y)", lambda x, y, z: math.sin(math.pi * x) * math.cos(2 * math.pi * y)),
+    ("tan(x) + atan(y) + atan2(y, x)", lambda x, y, z: math.tan(x) + math.atan(y) + math.atan2(y, x)),
+    ("asin(z/2) + acos(z/2)", lambda x, y, z: math.asin(z / 2) + math.acos(z / 2)),
+    ("sinh(x) - cosh(y) * tanh(z)", lambda x, y, z: math.sinh(x) - math.cosh(y) * math.tanh(z)),
+    ("log(abs(x) + 1) + log10(fabs(y) + 2)", lambda x, y, z: math.log(abs(x) + 1) + math.log10(abs(y) + 2)),
+    ("floor(10*x) + ceil(10*y)", lambda x, y, z: float(math.floor(10 * x) + math.ceil(10 * y))),
+    ("pow(x, 3) + min(x, y) + max(y, z)", lambda x, y, z: x ** 3.0 + min(x, y) + max(y, z)),
+    ("heaviside(x) - heaviside(-x)", lambda x, y, z: (1.0 if x > 0 else 0.0) - (1.0 if -x > 0 else 0.0)),
+    ("0.5*(1 + tanh((sqrt(x^2+y^2) - 0.3)/0.05))",
+     lambda x, y, z: 0.5 * (1 + math.tanh((math.sqrt(x ** 2 + y ** 2) - 0.3) / 0.05))),
+    (" x ^ 2 * 3 ", lambda x, y, z: x ** 2 * 3),
+]
+EXPR_ERRORS = ["x + (y", "w * 2", "foo(x)", "sin(x, y)", "pow(x)", "x +", "3 4"]
+
+
+def test_function_init_expressions_match_python(tmp_path):
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-0.9, 0.9, size=(6, 3))
+    lines, want = [], []
+    for text, fn in EXPR_CASES:
+        for x, y, z in pts.tolist():
+            lines.append(f"{x!r} {y!r} {z!r}\t{text}")
+            want.append(fn(x, y, z))
+    for text in EXPR_ERRORS:
+        lines.append(f"0 0 0\t{text}")
+    f = tmp_path / "exprs.txt"
+    f.write_text("\n".join(lines) + "\n")
+    out = _run([cpp_build.binary("dropin_host"), "expr", f]).stdout.splitlines()
+    assert len(out) == len(lines)
+    got = [float(v) for v in out[:len(want)]]
+    for g, w, line in zip(got, want, lines):
+        assert g == w, (line, g, w)
+    assert all(v.startswith("error ") for v in out[len(want):]), out[len(want):]
 
 
 # decorated registered right-hand sides, same ids as eval_case() in dropin_host.cpp
