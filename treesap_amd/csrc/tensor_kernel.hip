@@ -16,8 +16,9 @@
 // six h components in increasing order accumulates every sum in exactly the
 // reference's order: each accumulator receives at most one term per component.
 //
-// Tiling. A workgroup owns a 16 x 8 x 4 block of cells (one per thread) and
-// visits the components one at a time. For each it stages the block plus an
+// Tiling. A workgroup owns a 16 x 8 x 2 block of cells (one per thread; at
+// ~250 VGPRs two such workgroups share a CU, so one computes while the other
+// waits at a barrier) and visits the components one at a time. For each it stages the block plus an
 // H-wide halo of the (ghost-filled) source in LDS. The three unmixed second
 // derivatives and the KO terms are read from there. For the six mixed ones,
 // the inner first derivative g_i = D_i h is first evaluated at every point an
@@ -38,7 +39,7 @@ __device__ __forceinline__ double pick3(int m, double v0, double v1, double v2) 
 }
 
 template <int N, int R> struct TensorTile {
-    static constexpr int TX = 16, TY = 8, TZ = 4, NT = TX * TY * TZ;
+    static constexpr int TX = 16, TY = 8, TZ = 2, NT = TX * TY * TZ; // 256 threads: two workgroups per CU
     static constexpr int H = N > R ? N : R;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, REG = RX * RY * RZ;
 
@@ -336,7 +337,7 @@ template <int N, int R> struct TensorTile {
 };
 
 template <int N, int R>
-__global__ __launch_bounds__(512) void tensor_stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__(TensorTile<N, R>::NT) void tensor_stage_kernel(const StageArgs a) {
     __shared__ double lds[TensorTile<N, R>::LDS_DOUBLES];
     TensorTile<N, R>::run(a, lds);
 }
