@@ -337,7 +337,7 @@ template <int N, int R> struct TensorTile {
 };
 
 template <int N, int R>
-__global__ __launch_bounds__(TensorTile<N, R>::NT) void tensor_stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__((TensorTile<N, R>::NT)) void tensor_stage_kernel(const StageArgs a) {
     __shared__ double lds[TensorTile<N, R>::LDS_DOUBLES];
     TensorTile<N, R>::run(a, lds);
 }
