@@ -116,7 +116,11 @@ tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n);
  * library then creates its own default-priority stream). The descriptor is
  * validated before any HIP call; with an absorbing layer the device's
  * per-workgroup LDS is then queried, since the stage kernels keep the layer's
- * rate table (3 T^2 + 1 doubles) in LDS (TSP_UNSUPPORTED if it does not fit). */
+ * rate table (3 T^2 + 1 doubles) in LDS (TSP_UNSUPPORTED if it does not fit).
+ * Stage kernels address registers under 4 GiB through buffer descriptors and
+ * larger ones (up to the 2^31-element limit) through 64-bit pointers; setting
+ * the environment variable TSP_POINTER_ACCESS before tsp_create selects the
+ * pointer-access kernel for any size (results are bit-identical). */
 tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out);
 tsp_status tsp_destroy(tsp_solver *s);
 
