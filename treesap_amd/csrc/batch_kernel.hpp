@@ -60,7 +60,7 @@ template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
     using FS = FusedStage<Sys, N, R, TY, false, RS>;
     using NFS = FusedStage<Sys, N, R, 2 * TY, false, RS, true, 32>;
     static constexpr int LDS_TOTAL = FS::LDS_TOTAL > NFS::LDS_TOTAL ? FS::LDS_TOTAL : NFS::LDS_TOTAL;
-    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 120 * 1024;
+    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 144 * 1024;
 
     __device__ __forceinline__ static void run(const BatchInstance *__restrict__ inst,
                                                const BatchWork *__restrict__ work, const BatchArgs &b, double *lds,

@@ -249,10 +249,13 @@ inline bool regs_fit_buffer(const StageArgs &a, int ncomp) {
 }
 
 // RS > 0 (halo ring): the x/y halo of a plane is loaded RS+1 steps ahead of
-// its use, about when the neighbouring tiles load those cells as their own
-// z-queue heads (H+2 steps ahead) and L2 still holds them, and parked in an
-// LDS ring of RS slots until the plane is computed. Each thread stores and
-// reloads only its own halo cells, so the ring needs no barrier.
+// its use, close to when the neighbouring tiles load those cells as their own
+// z-queue heads (H+1 steps ahead) and L2 still holds them, and written at
+// once into that plane's buffer: the LDS holds a ring of RS+2 plane buffers
+// (plane t in buffer t mod (RS+2)). A buffer is rewritten at the top of step
+// t for plane t+RS; its previous plane, t-2, was last read during step t-2,
+// which every wave has finished once any wave is past the barrier of step
+// t-1. Without the ring (RS = 0) the two buffers alternate.
 //
 // TXW: tile width, 64 (one wave per x row) or 32 (each wave covers two rows;
 // for boxes whose x extent leaves a 32-wide remainder).
@@ -270,11 +273,11 @@ struct FusedStage {
     static constexpr int HPT = (NHALO + NT - 1) / NT;
     static constexpr int Q = 2 * H + 1;
     static constexpr int LDS_PLANE = NS * LH * LW;
-    static constexpr int LDS_DOUBLES = 2 * LDS_PLANE; // double-buffered plane
-    static constexpr int RING_DOUBLES = RS * NS * NHALO;
-    static constexpr int LDS_TOTAL = LDS_DOUBLES + RING_DOUBLES;
-    // ring variants are built where planes + ring stay within 120 KiB of LDS
-    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 120 * 1024;
+    static constexpr int NSLOT = RS > 0 ? RS + 2 : 2; // plane buffers
+    static constexpr int LDS_DOUBLES = NSLOT * LDS_PLANE;
+    static constexpr int LDS_TOTAL = LDS_DOUBLES;
+    // ring variants are built where the plane buffers stay within 144 KiB of LDS
+    static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 144 * 1024;
 
     using Plane = double[LH][LW];
 
@@ -420,9 +423,15 @@ struct FusedStage {
             });
         };
 
-        // halo slot m of component slot s for ring slot r (RS > 0 only)
-        double *ring = lds_raw + LDS_DOUBLES;
-        auto ring_at = [&](int r, int s, int m) -> double & { return ring[(r * NS + s) * NHALO + tid + m * NT]; };
+        // the halo of plane step p into its buffer
+        auto put_halo = [&](int slot) {
+            double *dst = lds_raw + slot * LDS_PLANE;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int m = 0; m < HPT; ++m)
+                    if (hok[m]) dst[s * LH * LW + hlds[m]] = hv[s][m];
+        };
 
         // prologue: the queue as it stands before step 0 advances it (planes
         // -H-1..H-1 of the chunk; the first is shifted out unread) and the
@@ -442,17 +451,14 @@ struct FusedStage {
 #pragma unroll
             for (int p = 0; p < RS; ++p) {
                 load_halo(min(p, tlast));
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-#pragma unroll
-                    for (int m = 0; m < HPT; ++m)
-                        if (hok[m]) ring_at(p, s, m) = hv[s][m];
+                put_halo(p);
             }
             load_halo(min(RS, tlast));
         } else {
             load_halo(0);
         }
-        int rslot = 0; // ring slot of step t: t % RS
+        int cslot = 0;                // buffer of plane step t: t % NSLOT
+        int hslot = RS % NSLOT;       // buffer of plane step t + RS
 
         for (int t = 0; t < cnt; ++t) {
             const int z = plane(t);
@@ -476,33 +482,20 @@ struct FusedStage {
                 if (read_acc) av[C] = mem.template acc<C>(ch, zp);
             });
             load_queue(min(t + 1, tlast));
-            // this plane into one half of the double-buffered LDS (the other
-            // half may still be read by waves finishing the previous plane)
-            double *buf = lds_raw + (t & 1) * LDS_PLANE;
+            // this plane's centre into its buffer, and the halo loaded during
+            // step t-1 (of plane t+RS; of plane t without the ring) into the
+            // buffer of its plane
+            double *buf = lds_raw + cslot * LDS_PLANE;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                buf[s * LH * LW + (row + H) * LW + lane + H] = q[s][H];
-#pragma unroll
-                for (int m = 0; m < HPT; ++m) {
-                    if (hok[m]) {
-                        if constexpr (RS > 0) {
-                            // the slot of step t takes the halo of step t+RS
-                            double &r = ring_at(rslot, s, m);
-                            const double hvt = r;
-                            r = hv[s][m];
-                            buf[s * LH * LW + hlds[m]] = hvt;
-                        } else {
-                            buf[s * LH * LW + hlds[m]] = hv[s][m];
-                        }
-                    }
-                }
-            }
+            for (int s = 0; s < NS; ++s) buf[s * LH * LW + (row + H) * LW + lane + H] = q[s][H];
+            put_halo(hslot);
             if constexpr (RS > 0) {
                 load_halo(min(t + RS + 1, tlast));
-                rslot = rslot == RS - 1 ? 0 : rslot + 1;
             } else {
                 load_halo(min(t + 1, tlast));
             }
+            cslot = cslot == NSLOT - 1 ? 0 : cslot + 1;
+            hslot = hslot == NSLOT - 1 ? 0 : hslot + 1;
             __syncthreads();
 
             if (active) {
@@ -588,9 +581,10 @@ __global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a)
     FusedStage<Sys, N, R, TY, ADJ, RS, BUF>::run(a);
 }
 
-// Ring slots of the halo-ring variant by tile height: H+1 for 64x16 tiles (one
-// block per CU), H for 64x8 tiles (so that two blocks share a CU's LDS).
-template <int TY, int H> constexpr int ring_slots() { return TY == 16 ? H + 1 : TY == 8 ? H : 0; }
+// Halo lookahead of the ring variant by tile height: 3 planes for 64x16 tiles
+// (one block per CU; five plane buffers), 2 for 64x8 tiles (four buffers, so
+// that two blocks share a CU's LDS).
+template <int TY, int H> constexpr int ring_slots() { return TY == 16 ? 3 : TY == 8 ? 2 : 0; }
 
 // LDS a block may hold (static + dynamic), queried once per device
 inline size_t max_block_lds() {
