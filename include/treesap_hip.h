@@ -92,7 +92,8 @@ typedef struct tsp_desc {
  * configuration produces bit-identical results; only the speed differs. */
 typedef struct tsp_kernel_cfg {
     int32_t ghost_mode; /* 0: virtual ghosts (no fill pass); 1: FillPatch pass then plain reads */
-    int32_t tile_y;     /* rows (waves) per workgroup: 2, 4, 8 or 16; tile_x is 64 */
+    int32_t tile_y;     /* rows per workgroup: 2, 4, 8 or 16 with 64-wide tiles (one wave per row),
+                           or 32 for 32 x 32 tiles (two rows per wave) */
     int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
     int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
     int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together) */

@@ -99,6 +99,8 @@ TILE_CFGS = [
     {"ghost_mode": 1, "tile_y": 4, "zchunk": 5, "xcd_remap": 1, "zalt": 1, "halo_ring": 0},
     {"ghost_mode": 1, "tile_y": 8, "zchunk": 1, "xcd_remap": 0, "zalt": 0, "halo_ring": 1},
     {"ghost_mode": 0, "tile_y": 8, "zchunk": 2, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
+    {"ghost_mode": 0, "tile_y": 32, "zchunk": 6, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
+    {"ghost_mode": 1, "tile_y": 32, "zchunk": 23, "xcd_remap": 0, "zalt": 0, "halo_ring": 0},
 ]
 
 
@@ -230,7 +232,7 @@ def test_autotune_restores_state():
     s.rk4_steps(2)
     before = s.get_state()
     cfg = s.autotune(1)
-    assert cfg["tile_y"] in (2, 4, 8)
+    assert cfg["tile_y"] in (2, 4, 8, 16, 32)
     assert s.steps == 2
     assert np.array_equal(s.get_state(), before)
     s.rk4_steps(3)

@@ -81,8 +81,8 @@ double *reg_origin(const tsp_solver *s, int reg) {
 tsp_status validate_cfg(const tsp_kernel_cfg *c) {
     if (!c) return fail(TSP_INVALID_ARG, "kernel config is NULL");
     if (c->ghost_mode != 0 && c->ghost_mode != 1) return fail(TSP_INVALID_ARG, "ghost_mode must be 0 or 1");
-    if (c->tile_y != 2 && c->tile_y != 4 && c->tile_y != 8 && c->tile_y != 16)
-        return fail(TSP_INVALID_ARG, "tile_y must be 2, 4, 8 or 16");
+    if (c->tile_y != 2 && c->tile_y != 4 && c->tile_y != 8 && c->tile_y != 16 && c->tile_y != 32)
+        return fail(TSP_INVALID_ARG, "tile_y must be 2, 4, 8, 16 or 32");
     if (c->zchunk < 1) return fail(TSP_INVALID_ARG, "zchunk must be >= 1");
     if (c->xcd_remap != 0 && c->xcd_remap != 1) return fail(TSP_INVALID_ARG, "xcd_remap must be 0 or 1");
     if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
@@ -648,7 +648,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     // (for the tile heights that have one) halo ring with virtual ghosts, then
     // the best geometry with the fill-pass mode
     std::vector<tsp_kernel_cfg> cands;
-    for (int ty : {2, 4, 8, 16})
+    for (int ty : {2, 4, 8, 16, 32})
         for (int zc : zchunks)
             for (int xr = 0; xr < 2; ++xr)
                 for (int za = 0; za < 2; ++za)

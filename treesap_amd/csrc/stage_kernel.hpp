@@ -576,15 +576,15 @@ struct FusedStage {
 // dynamic LDS bytes for the ABL rate table of thickness T
 inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
-template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true>
-__global__ __launch_bounds__(64 * TY) void fused_stage_kernel(const StageArgs a) {
-    FusedStage<Sys, N, R, TY, ADJ, RS, BUF>::run(a);
+template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true, int TXW = 64>
+__global__ __launch_bounds__(TXW * TY) void fused_stage_kernel(const StageArgs a) {
+    FusedStage<Sys, N, R, TY, ADJ, RS, BUF, TXW>::run(a);
 }
 
 // Halo lookahead of the ring variant by tile height: 3 planes for 64x16 tiles
 // (one block per CU; five plane buffers), 2 for 64x8 tiles (four buffers, so
 // that two blocks share a CU's LDS).
-template <int TY, int H> constexpr int ring_slots() { return TY == 16 ? 3 : TY == 8 ? 2 : 0; }
+template <int TY, int H> constexpr int ring_slots() { return TY >= 16 ? 3 : TY == 8 ? 2 : 0; }
 
 // LDS a block may hold (static + dynamic), queried once per device
 inline size_t max_block_lds() {
@@ -603,17 +603,18 @@ inline size_t max_block_lds() {
 
 // Tile heights with a halo-ring variant: use it when asked for and when its
 // LDS (static + the dynamic ABL table) fits a block on this device.
-template <class Sys, int N, int R, int TY, bool ADJ>
+template <class Sys, int N, int R, int TY, bool ADJ, int TXW = 64>
 hipError_t launch_tile(const StageArgs &a, dim3 grid, size_t shm, hipStream_t stream) {
     constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY, ADJ>::H>();
-    using Ring = FusedStage<Sys, N, R, TY, ADJ, RS>;
+    using Ring = FusedStage<Sys, N, R, TY, ADJ, RS, true, TXW>;
     if constexpr (RS > 0 && Ring::FITS) {
         if (a.halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
-            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, RS>), grid, dim3(64 * TY), shm, stream, a);
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, RS, true, TXW>), grid, dim3(TXW * TY), shm,
+                               stream, a);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ>), grid, dim3(64 * TY), shm, stream, a);
+    hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, 0, true, TXW>), grid, dim3(TXW * TY), shm, stream, a);
     return hipGetLastError();
 }
 
@@ -623,7 +624,8 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     // registers of 4 GiB or more run one pointer-access kernel (64 x 8 tiles)
     const bool buf = !a.ptr_access && regs_fit_buffer(a, Sys::NC);
     if (!buf) tile_y = 8;
-    a.tiles_x = (a.nx + 63) / 64;
+    const int txw = tile_y == 32 ? 32 : 64; // tile_y 32: 32 x 32 tiles, two rows per wave
+    a.tiles_x = (a.nx + txw - 1) / txw;
     a.tiles_y = (a.ny + tile_y - 1) / tile_y;
     a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
     const long long nblocks = (long long)a.tiles_x * a.tiles_y * a.nzc;
@@ -639,6 +641,7 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a); break;
     case 8: return launch_tile<Sys, N, R, 8, ADJ>(a, grid, shm, stream);
     case 16: return launch_tile<Sys, N, R, 16, ADJ>(a, grid, shm, stream);
+    case 32: return launch_tile<Sys, N, R, 32, ADJ, 32>(a, grid, shm, stream);
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
