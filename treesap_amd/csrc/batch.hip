@@ -18,14 +18,12 @@
 #include <vector>
 
 namespace tsp {
+// forward stages 0 and 3 run their stage-specific bodies
 hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                 hipStream_t stream) {
-    hipError_t e = hipErrorInvalidValue;
-    static_for<0, kBatchVariants>([&]<int V>() {
-        if (variant == V)
-            e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V]>(b, tile_y, grid, max_abl_T, halo_ring, stream);
-    });
-    return e;
+    if (b.stage == kStage0) return launch_batched_phipi_s0(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
+    if (b.stage == kStage3) return launch_batched_phipi_s3(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
+    return launch_batched_phipi_mode<-1>(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
 }
 } // namespace tsp
 

@@ -56,9 +56,10 @@ inline int batch_variant(int n, int r) {
 // One (stencil width, KO order) group: a persistent grid strides over the
 // group's work items. RS > 0: the halo-ring tile body. Items are 64 x TY
 // tiles or, flagged narrow, 32 x 2*TY tiles (same thread count).
-template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
-    using FS = FusedStage<Sys, N, R, TY, false, RS>;
-    using NFS = FusedStage<Sys, N, R, 2 * TY, false, RS, true, 32>;
+// SM: compile-time stage kind (forward stage 0 or 3) or -1 (from the arguments).
+template <class Sys, int N, int R, int TY, int RS, int SM = -1> struct BatchedStage {
+    using FS = FusedStage<Sys, N, R, TY, false, RS, true, 64, SM>;
+    using NFS = FusedStage<Sys, N, R, 2 * TY, false, RS, true, 32, SM>;
     static constexpr int LDS_TOTAL = FS::LDS_TOTAL > NFS::LDS_TOTAL ? FS::LDS_TOTAL : NFS::LDS_TOTAL;
     static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 144 * 1024;
 
@@ -130,42 +131,60 @@ template <class Sys, int N, int R, int TY, int RS> struct BatchedStage {
 // The instance table and work list are separate restrict-qualified arguments:
 // nothing the kernel stores can alias them, so their uniform loads can be
 // scalar loads.
-template <class Sys, int N, int R, int TY, int RS = 0>
+template <class Sys, int N, int R, int TY, int RS = 0, int SM = -1>
 __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInstance *__restrict__ inst,
                                                                 const BatchWork *__restrict__ work,
                                                                 const BatchArgs b) {
-    __shared__ double lds[BatchedStage<Sys, N, R, TY, RS>::LDS_TOTAL];
+    __shared__ double lds[BatchedStage<Sys, N, R, TY, RS, SM>::LDS_TOTAL];
     extern __shared__ double sig_lds[];
-    BatchedStage<Sys, N, R, TY, RS>::run(inst, work, b, lds, sig_lds);
+    BatchedStage<Sys, N, R, TY, RS, SM>::run(inst, work, b, lds, sig_lds);
 }
 
 // one tile height: the halo-ring kernel when asked for and its LDS fits
-template <class Sys, int N, int R, int TY>
+template <class Sys, int N, int R, int TY, int SM>
 hipError_t launch_batched_tile(const BatchArgs &b, int grid, size_t shm, int halo_ring, hipStream_t stream) {
     constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY>::H>();
-    using Ring = BatchedStage<Sys, N, R, TY, RS>;
+    using Ring = BatchedStage<Sys, N, R, TY, RS, SM>;
     if constexpr (RS > 0 && Ring::FITS) {
         if (halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
-            hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS>), dim3(grid), dim3(64 * TY), shm, stream,
+            hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS, SM>), dim3(grid), dim3(64 * TY), shm, stream,
                                b.inst, b.work, b);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY>), dim3(grid), dim3(64 * TY), shm, stream, b.inst, b.work,
-                       b);
+    hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, 0, SM>), dim3(grid), dim3(64 * TY), shm, stream, b.inst,
+                       b.work, b);
     return hipGetLastError();
 }
 
 // `max_abl_T`: the thickest absorbing layer in the group (sizes the LDS rate table)
-template <class Sys, int N, int R>
+template <class Sys, int N, int R, int SM>
 hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring, hipStream_t stream) {
     const size_t shm = sigma_lds_bytes(max_abl_T);
     switch (tile_y) {
-    case 8: return launch_batched_tile<Sys, N, R, 8>(b, grid, shm, halo_ring, stream);
-    case 16: return launch_batched_tile<Sys, N, R, 16>(b, grid, shm, halo_ring, stream);
+    case 8: return launch_batched_tile<Sys, N, R, 8, SM>(b, grid, shm, halo_ring, stream);
+    case 16: return launch_batched_tile<Sys, N, R, 16, SM>(b, grid, shm, halo_ring, stream);
     default: return hipErrorInvalidValue;
     }
 }
+
+// every (stencil width, KO order) variant of the (phi, Pi) batch for one
+// stage body; instantiated in batch.hip (SM = -1), batch_s0.hip and
+// batch_s3.hip, which the build compiles in parallel
+template <int SM>
+hipError_t launch_batched_phipi_mode(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                     int halo_ring, hipStream_t stream) {
+    hipError_t e = hipErrorInvalidValue;
+    static_for<0, kBatchVariants>([&]<int V>() {
+        if (variant == V)
+            e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V], SM>(b, tile_y, grid, max_abl_T, halo_ring, stream);
+    });
+    return e;
+}
+hipError_t launch_batched_phipi_s0(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                                   hipStream_t stream);
+hipError_t launch_batched_phipi_s3(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                                   hipStream_t stream);
 
 // launch one variant group of the (phi, Pi) batch
 hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,

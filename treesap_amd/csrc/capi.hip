@@ -45,6 +45,7 @@ struct tsp_solver {
     bool ptr_access = false;
     double *sigma_tab = nullptr;
     tsp::LaunchFn launch = nullptr;
+    tsp::LaunchFn launch_stage[4] = {nullptr, nullptr, nullptr, nullptr}; // stage-specific, where built
     tsp_kernel_cfg cfg{0, 8, 64, 1, 1, 1};
     double time = 0.0;
     int64_t steps = 0;
@@ -189,7 +190,10 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
         const int src = stage == 0 ? OLD : ARG;
         const tsp::StageArgs a = stage_args(s, stage, src, OLD, NEW, stage < 3 ? ARGN : -1, b, a_next);
         if (tm) TSP_HIP(hipEventRecord(tm->stage[stage].start, s->stream));
-        TSP_HIP(s->launch(a, s->cfg.tile_y, s->stream));
+        hipError_t e = hipErrorNotSupported;
+        if (s->launch_stage[stage]) e = s->launch_stage[stage](a, s->cfg.tile_y, s->stream);
+        if (e == hipErrorNotSupported) e = s->launch(a, s->cfg.tile_y, s->stream);
+        TSP_HIP(e);
         if (tm) TSP_HIP(hipEventRecord(tm->stage[stage].stop, s->stream));
     }
     std::swap(OLD, NEW); // post_step
@@ -260,6 +264,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     s->sc = s->sk * (d->n[2] + 2 * s->G);
     s->reg_elems = s->sc * s->nc;
     s->launch = lookup_launcher(d->system, d->stencil_n, d->ko_order);
+    for (int st = 0; st < 4; ++st) s->launch_stage[st] = lookup_stage_launcher(d->system, d->stencil_n, d->ko_order, st);
     s->cfg = default_cfg(*d);
     s->ptr_access = std::getenv("TSP_POINTER_ACCESS") != nullptr;
     // Geometry::CellSize = (prob_hi - prob_lo) / n_cells; grid factors as the stencils form them

@@ -141,6 +141,14 @@ LaunchFn lookup_phipi_adjoint(int n, int r);
 LaunchFn lookup_phiu(int n, int r);
 LaunchFn lookup_nonlinear(int n, int r);
 LaunchFn lookup_tensor(int n, int r);
+// launchers specialised for forward RK4 stage 0 / stage 3 (hipErrorNotSupported
+// for configurations they do not cover: the caller then uses the generic one)
+LaunchFn lookup_phipi_s0(int n, int r);
+LaunchFn lookup_phipi_s3(int n, int r);
+LaunchFn lookup_phiu_s0(int n, int r);
+LaunchFn lookup_phiu_s3(int n, int r);
+LaunchFn lookup_nonlinear_s0(int n, int r);
+LaunchFn lookup_nonlinear_s3(int n, int r);
 
 // Ghost fill (SingleLevelIntegrator.hpp:153-163) of one register.
 struct GhostArgs {

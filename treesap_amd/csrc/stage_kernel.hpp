@@ -259,7 +259,12 @@ inline bool regs_fit_buffer(const StageArgs &a, int ncomp) {
 //
 // TXW: tile width, 64 (one wave per x row) or 32 (each wave covers two rows;
 // for boxes whose x extent leaves a 32-wide remainder).
-template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BUF = true, int TXW = 64>
+//
+// SM: the stage kind as a compile-time constant (kStage0, kStage3 of the
+// forward RK4), or -1 for one body that reads it from the arguments. A fixed
+// stage drops the loads and stores it does not need from the code, not just
+// at run time (forward stage 0 +8 %, stage 3 +16 % on wave256).
+template <class Sys, int N, int R, int TY, bool ADJ = false, int RS = 0, bool BUF = true, int TXW = 64, int SM = -1>
 struct FusedStage {
     static_assert(TXW == 64 || TXW == 32, "tile width 64 or 32");
     static constexpr int TX = TXW;
@@ -348,7 +353,7 @@ struct FusedStage {
         // the last step's planes, which the caches hold, instead of new ones
         const int tlast = cnt - 1;
 
-        const int stage = a.stage;
+        const int stage = SM >= 0 ? SM : a.stage;
         bool is_first, write_argn, read_init;
         if constexpr (ADJ) {
             is_first = stage == kStage3;
@@ -576,9 +581,9 @@ struct FusedStage {
 // dynamic LDS bytes for the ABL rate table of thickness T
 inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
-template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true, int TXW = 64>
+template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true, int TXW = 64, int SM = -1>
 __global__ __launch_bounds__(TXW * TY) void fused_stage_kernel(const StageArgs a) {
-    FusedStage<Sys, N, R, TY, ADJ, RS, BUF, TXW>::run(a);
+    FusedStage<Sys, N, R, TY, ADJ, RS, BUF, TXW, SM>::run(a);
 }
 
 // Halo lookahead of the ring variant by tile height: 3 planes for 64x16 tiles
@@ -603,26 +608,33 @@ inline size_t max_block_lds() {
 
 // Tile heights with a halo-ring variant: use it when asked for and when its
 // LDS (static + the dynamic ABL table) fits a block on this device.
-template <class Sys, int N, int R, int TY, bool ADJ, int TXW = 64>
+template <class Sys, int N, int R, int TY, bool ADJ, int TXW = 64, int SM = -1>
 hipError_t launch_tile(const StageArgs &a, dim3 grid, size_t shm, hipStream_t stream) {
     constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY, ADJ>::H>();
-    using Ring = FusedStage<Sys, N, R, TY, ADJ, RS, true, TXW>;
+    using Ring = FusedStage<Sys, N, R, TY, ADJ, RS, true, TXW, SM>;
     if constexpr (RS > 0 && Ring::FITS) {
         if (a.halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
-            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, RS, true, TXW>), grid, dim3(TXW * TY), shm,
-                               stream, a);
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, RS, true, TXW, SM>), grid, dim3(TXW * TY),
+                               shm, stream, a);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, 0, true, TXW>), grid, dim3(TXW * TY), shm, stream, a);
+    hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, TY, ADJ, 0, true, TXW, SM>), grid, dim3(TXW * TY), shm, stream,
+                       a);
     return hipGetLastError();
 }
 
-template <class Sys, int N, int R, bool ADJ = false>
+// SM >= 0: stage-specific kernels for the 64x8, 64x16 and 32x32 tiles; for
+// the other tile heights and the pointer-access kernel it returns
+// hipErrorNotSupported without launching, and the caller uses the generic
+// launcher (SM = -1).
+template <class Sys, int N, int R, bool ADJ = false, int SM = -1>
 hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     StageArgs a = a0;
     // registers of 4 GiB or more run one pointer-access kernel (64 x 8 tiles)
     const bool buf = !a.ptr_access && regs_fit_buffer(a, Sys::NC);
+    if constexpr (SM >= 0)
+        if (!buf || (tile_y != 8 && tile_y != 16 && tile_y != 32)) return hipErrorNotSupported;
     if (!buf) tile_y = 8;
     const int txw = tile_y == 32 ? 32 : 64; // tile_y 32: 32 x 32 tiles, two rows per wave
     a.tiles_x = (a.nx + txw - 1) / txw;
@@ -632,25 +644,33 @@ hipError_t launch_fused(const StageArgs &a0, int tile_y, hipStream_t stream) {
     if (nblocks <= 0 || nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)nblocks);
     const size_t shm = sigma_lds_bytes(a.abl_T);
-    if (!buf) {
-        hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ, 0, false>), grid, dim3(512), shm, stream, a);
-        return hipGetLastError();
+    if constexpr (SM < 0) {
+        if (!buf) {
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 8, ADJ, 0, false>), grid, dim3(512), shm, stream, a);
+            return hipGetLastError();
+        }
+        if (tile_y == 2) {
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), shm, stream, a);
+            return hipGetLastError();
+        }
+        if (tile_y == 4) {
+            hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a);
+            return hipGetLastError();
+        }
     }
     switch (tile_y) {
-    case 2: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 2, ADJ>), grid, dim3(128), shm, stream, a); break;
-    case 4: hipLaunchKernelGGL((fused_stage_kernel<Sys, N, R, 4, ADJ>), grid, dim3(256), shm, stream, a); break;
-    case 8: return launch_tile<Sys, N, R, 8, ADJ>(a, grid, shm, stream);
-    case 16: return launch_tile<Sys, N, R, 16, ADJ>(a, grid, shm, stream);
-    case 32: return launch_tile<Sys, N, R, 32, ADJ, 32>(a, grid, shm, stream);
+    case 8: return launch_tile<Sys, N, R, 8, ADJ, 64, SM>(a, grid, shm, stream);
+    case 16: return launch_tile<Sys, N, R, 16, ADJ, 64, SM>(a, grid, shm, stream);
+    case 32: return launch_tile<Sys, N, R, 32, ADJ, 32, SM>(a, grid, shm, stream);
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 // Supported (stencil width, KO order) pairs: r in {0, n, n+1} with r <= 4.
-template <class Sys, bool ADJ = false> LaunchFn lookup_fused(int n, int r) {
+template <class Sys, bool ADJ = false, int SM = -1> LaunchFn lookup_fused(int n, int r) {
 #define TSP_CASE(NN, RR)                                                                                              \
-    if (n == NN && r == RR) return &launch_fused<Sys, NN, RR, ADJ>;
+    if (n == NN && r == RR) return &launch_fused<Sys, NN, RR, ADJ, SM>;
     TSP_CASE(1, 0)
     TSP_CASE(1, 1)
     TSP_CASE(1, 2)
