@@ -9,16 +9,18 @@
 // KODecorator.hpp:58-70), then the stage axpys.
 //
 // Layout on the GPU (gfx950, wave64):
-//   * a workgroup owns a 64 x TY column tile in (x, y) and marches a chunk of
-//     z planes; wave w of the group owns x-row y0+w, lane l owns x0+l, so every
-//     global load and store of a wave is one contiguous 512-byte row;
+//   * a workgroup owns a 64 x TY column tile in (x, y) (or 32 x 32, two rows
+//     per wave) and marches a chunk of z planes; wave w of the group owns
+//     x-row y0+w, lane l owns x0+l, so every global load and store of a wave
+//     is one contiguous 512-byte row;
 //   * z neighbours live in a per-lane register queue of 2H+1 planes (each
 //     plane of the source register is loaded from HBM once per chunk, one
-//     plane ahead of use);
+//     step ahead of use);
 //   * x and y neighbours come from an LDS copy of the current plane with an
 //     H-wide halo along x and y (corners are not needed for axis stencils);
-//     the halo of the next plane is prefetched into registers while the
-//     current plane is computed;
+//     the halo of the next plane (or, with the halo ring, of a plane RS steps
+//     further) is prefetched into registers while the current plane is
+//     computed;
 //   * ghosts are either virtual (out-of-domain indices are mapped: periodic
 //     wrap, Dirichlet constant) or read from ghost memory after a fill pass.
 // The arithmetic is the reference's expression for expression (weights times
@@ -334,12 +336,12 @@ struct FusedStage {
     // Memory pipeline. While plane t is computed, the x/y halo of plane t+1
     // (t+RS+1 with the ring) and the z-queue head of plane t+H+1 are in
     // flight; the init/acc values of plane t are issued at the top of its step
-    // and consumed at its end. The
-    // loads that carry values into the next step are unconditional (their
-    // indices are clamped into the allocation; values beyond the chunk are
-    // never used), so each one lands in a fixed register and no copy forces
-    // an early wait. The memory counter completes in order, so every wait
-    // targets a load issued a phase earlier and leaves the newer ones in flight.
+    // and consumed at its end. The loads that carry values into the next step
+    // are unconditional (their indices are clamped into the allocation; values
+    // beyond the chunk are never used), so each one lands in a fixed register
+    // and no copy forces an early wait. The memory counter completes in order,
+    // so every wait targets a load issued a phase earlier and leaves the newer
+    // ones in flight.
     template <bool DOWN>
     __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
                                                     double *lds_raw, const double *sig) {
@@ -378,7 +380,7 @@ struct FusedStage {
         const int ccol = active ? gi + gj * a.sj : qcol; // centre column (clamped for lanes outside the box)
         const auto qh = mem.col(qcol), ch = mem.col(ccol);
         decltype(mem.col(0)) hh[HPT];
-        int hcol[HPT], hlds[HPT];
+        int hlds[HPT];
         bool hok[HPT];
 #pragma unroll
         for (int m = 0; m < HPT; ++m) {
@@ -395,8 +397,9 @@ struct FusedStage {
                 hx = mm < H ? mm - H : TX + (mm - H);
             }
             hok[m] = h < NHALO;
-            hcol[m] = hok[m] ? map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj : qcol;
-            hh[m] = mem.col(hcol[m]);
+            const int hcol =
+                hok[m] ? map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj : qcol;
+            hh[m] = mem.col(hcol);
             hlds[m] = (hy + H) * LW + hx + H;
         }
 
@@ -441,7 +444,7 @@ struct FusedStage {
         // prologue: the queue as it stands before step 0 advances it (planes
         // -H-1..H-1 of the chunk; the first is shifted out unread) and the
         // queue head of step 0 in flight; the halo of step 0 in flight, or with
-        // the ring, the halos of steps 0..RS-1 parked and that of step RS in flight
+        // the ring, the halos of steps 0..RS-1 in their buffers and that of step RS in flight
         static_for<0, NC>([&]<int C>() {
             if constexpr (Staging<Sys, R>::staged(C)) {
                 constexpr int sl = Staging<Sys, R>::slot(C);
