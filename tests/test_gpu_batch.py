@@ -97,3 +97,39 @@ def test_small_ensemble_workload():
         o.set_state(st)
         o.rk4_steps(20)
         assert np.array_equal(b.get_state(i), o.get_state()), i
+
+
+# (stencil width, KO order) pairs of the batched kernel
+_BATCH_PAIRS = [(1, 0), (1, 1), (2, 0), (2, 2), (2, 3), (3, 0), (3, 3), (4, 0), (4, 4)]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batches_match_oracle(seed):
+    """Seeded random batches: 5-9 (phi, Pi) instances of random size, stencil/KO pair, boundaries and time
+    step under a random batch configuration (tile height, z chunk, grid, march order, halo ring, narrow
+    tiles); every instance bitwise against the oracle after three RK4 steps."""
+    rng = np.random.default_rng(2000 + seed)
+    cases = []
+    for k in range(int(rng.integers(5, 10))):
+        sn, ko = _BATCH_PAIRS[int(rng.integers(len(_BATCH_PAIRS)))]
+        H = max(sn, ko)
+        n = (int(rng.integers(max(H, 5), 110)), int(rng.integers(max(H, 3), 40)), int(rng.integers(max(H, 3), 24)))
+        per = tuple(int(v) for v in rng.integers(0, 2, 3))
+        abl = int(rng.integers(1, min(n) // 2)) if min(n) >= 6 and rng.random() < 0.4 else 0
+        case, state = small_case("phipi", n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=300 + 10 * seed + k)
+        cases.append((case.replace(dt=case.dt * (1.0 + 0.05 * k)), state))
+    # tile height, halo ring and narrow tiles cover their combinations over the seeds; the rest is random
+    cfg = {"tile_y": [8, 16][seed % 2], "zchunk": int(rng.choice([0, 1, 5, 64])),
+           "grid_blocks": int(rng.choice([0, 7, 300])), "zalt": int(rng.integers(2)),
+           "halo_ring": [1, 1, 0, 1, 0, 1][seed], "narrow_tiles": [1, 1, 1, 0, 0, 1][seed]}
+    _say(f"[batch] random seed={seed} {len(cases)} instances {[c.n for c, _ in cases]} cfg={cfg}")
+    b = Batch([c for c, _ in cases])
+    b.kernel = cfg
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.rk4_steps(3)
+    for i, (case, st) in enumerate(cases):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(3)
+        assert np.array_equal(b.get_state(i), o.get_state()), (i, case.n)
