@@ -31,7 +31,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_FIELDS = ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt", "halo_ring")
+KERNEL_FIELDS = ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt", "halo_ring", "stage_pairs")
 
 
 def log(*a):
@@ -44,7 +44,9 @@ def traffic_from_profiles(cfg: dict):
     p = ROOT / "profiles" / "fused_stage_wave256_traffic.json"
     if not p.exists():
         return None
-    key = ",".join(str(cfg[k]) for k in KERNEL_FIELDS)
+    # single-stage configurations are keyed by their first six fields
+    fields = KERNEL_FIELDS if cfg.get("stage_pairs") else KERNEL_FIELDS[:6]
+    key = ",".join(str(cfg[k]) for k in fields)
     try:
         return json.loads(p.read_text())["configs"].get(key, {}).get("hbm_bytes_per_launch")
     except (ValueError, OSError, KeyError):

@@ -99,6 +99,13 @@ typedef struct tsp_kernel_cfg {
     int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together) */
     int32_t halo_ring;  /* 1: tile_y 8/16 load each plane's x/y halo with its z-queue head and keep
                            it in LDS until use (falls back to 0 where the LDS does not fit) */
+    int32_t stage_pairs; /* 1: RK4 stages (0,1) and (2,3) each run as one launch on 32 x 32 tiles
+                            (the first stage of a pair also on an H-wide ring of neighbour cells;
+                            tile_y, zalt and halo_ring do not apply). (phi, Pi) system with KO
+                            dissipation, ghost_mode 0, registers under 4 GiB and periodic axes of
+                            at least 2*max(stencil_n, ko_order) cells; otherwise tsp_set_kernel
+                            returns TSP_UNSUPPORTED. The intermediate stage registers (roles arg
+                            and arg_next) are not written in this mode. */
 } tsp_kernel_cfg;
 
 typedef struct tsp_solver tsp_solver;

@@ -60,6 +60,7 @@ class TspKernelCfg(C.Structure):
         ("xcd_remap", C.c_int32),
         ("zalt", C.c_int32),
         ("halo_ring", C.c_int32),
+        ("stage_pairs", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -46,7 +46,8 @@ struct tsp_solver {
     double *sigma_tab = nullptr;
     tsp::LaunchFn launch = nullptr;
     tsp::LaunchFn launch_stage[4] = {nullptr, nullptr, nullptr, nullptr}; // stage-specific, where built
-    tsp_kernel_cfg cfg{0, 8, 64, 1, 1, 1};
+    tsp::PairFn launch_pair = nullptr;                                      // stage pairs, where built
+    tsp_kernel_cfg cfg{0, 8, 64, 1, 1, 1, 0};
     double time = 0.0;
     int64_t steps = 0;
     double gf1[3]{}, gf2[3]{};
@@ -61,7 +62,7 @@ namespace {
 // tall enough, and the longest z chunk (of 64, 32, 16, 8) that still gives at
 // least two workgroups per CU, so small boxes fill the device too.
 tsp_kernel_cfg default_cfg(const tsp_desc &d) {
-    tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1};
+    tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1, 0};
     if (d.system == TSP_SYS_TENSOR_ADM) c.ghost_mode = 1; // mixed stencils read filled edge/corner ghosts
     const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
     for (int zc : {64, 32, 16, 8}) {
@@ -88,7 +89,24 @@ tsp_status validate_cfg(const tsp_kernel_cfg *c) {
     if (c->xcd_remap != 0 && c->xcd_remap != 1) return fail(TSP_INVALID_ARG, "xcd_remap must be 0 or 1");
     if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
     if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
+    if (c->stage_pairs != 0 && c->stage_pairs != 1) return fail(TSP_INVALID_ARG, "stage_pairs must be 0 or 1");
     return TSP_OK;
+}
+
+// Whether the stage-pair kernels apply to this solver with ghost mode
+// `ghost_mode` (the launcher re-checks what depends on the device).
+bool pairs_apply(const tsp_solver *s, int ghost_mode, std::string *why) {
+    auto no = [&](const char *m) {
+        if (why) *why = m;
+        return false;
+    };
+    if (!s->launch_pair) return no("stage_pairs needs the (phi, Pi) system with KO dissipation");
+    if (ghost_mode != 0) return no("stage_pairs needs ghost_mode 0");
+    if (s->ptr_access || (int64_t)sizeof(double) * s->reg_elems >= (int64_t(1) << 32))
+        return no("stage_pairs needs registers under 4 GiB");
+    for (int a = 0; a < 3; ++a)
+        if (s->d.periodic[a] && s->d.n[a] < 2 * s->H) return no("stage_pairs needs periodic axes of at least 2H cells");
+    return true;
 }
 
 void drop_graphs(tsp_solver *s) {
@@ -179,6 +197,37 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
     };
     tsp_status st = fill(OLD, 0); // pre_step
     if (st != TSP_OK) return st;
+    if (s->cfg.stage_pairs && s->launch_pair) {
+        // stages (0,1) and (2,3): the registers the four single stages would
+        // use, with the intermediate argument registers read at ghosts only
+        const int ra = ARG, rb = ARGN;
+        tsp::StageArgs p0 = stage_args(s, 0, OLD, rb, NEW, ra, dt * kBFrac[0], dt * kAFrac[1]);
+        p0.b1 = dt * kBFrac[1];
+        p0.a_next1 = dt * kAFrac[2];
+        tsp::StageArgs p2 = stage_args(s, 2, ra, OLD, NEW, rb, dt * kBFrac[2], dt * kAFrac[3]);
+        p2.b1 = dt * kBFrac[3];
+        p2.a_next1 = 0.0;
+        if (tm) TSP_HIP(hipEventRecord(tm->stage[0].start, s->stream));
+        hipError_t e = s->launch_pair(p0, 0, s->stream);
+        if (e != hipErrorNotSupported) { // otherwise nothing was launched: run the single stages
+            TSP_HIP(e);
+            if (tm) {
+                TSP_HIP(hipEventRecord(tm->stage[0].stop, s->stream));
+                TSP_HIP(hipEventRecord(tm->stage[1].start, s->stream));
+                TSP_HIP(hipEventRecord(tm->stage[1].stop, s->stream));
+                TSP_HIP(hipEventRecord(tm->stage[2].start, s->stream));
+            }
+            TSP_HIP(s->launch_pair(p2, 2, s->stream));
+            if (tm) {
+                TSP_HIP(hipEventRecord(tm->stage[2].stop, s->stream));
+                TSP_HIP(hipEventRecord(tm->stage[3].start, s->stream));
+                TSP_HIP(hipEventRecord(tm->stage[3].stop, s->stream));
+            }
+            std::swap(ARG, ARGN); // three swaps of the single-stage sequence
+            std::swap(OLD, NEW);
+            return TSP_OK;
+        }
+    }
     for (int stage = 0; stage < 4; ++stage) {
         const double a_next = dt * kAFrac[(stage + 1) % 4];
         const double b = dt * kBFrac[stage];
@@ -265,6 +314,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     s->reg_elems = s->sc * s->nc;
     s->launch = lookup_launcher(d->system, d->stencil_n, d->ko_order);
     for (int st = 0; st < 4; ++st) s->launch_stage[st] = lookup_stage_launcher(d->system, d->stencil_n, d->ko_order, st);
+    if (d->system == TSP_SYS_WAVE_PHIPI) s->launch_pair = tsp::lookup_pair_phipi(d->stencil_n, d->ko_order);
     s->cfg = default_cfg(*d);
     s->ptr_access = std::getenv("TSP_POINTER_ACCESS") != nullptr;
     // Geometry::CellSize = (prob_hi - prob_lo) / n_cells; grid factors as the stencils form them
@@ -441,6 +491,8 @@ tsp_status tsp_set_kernel(tsp_solver *s, const tsp_kernel_cfg *cfg) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     tsp_status st = validate_cfg(cfg);
     if (st != TSP_OK) return st;
+    std::string why;
+    if (cfg->stage_pairs && !pairs_apply(s, cfg->ghost_mode, &why)) return fail(TSP_UNSUPPORTED, why);
     s->cfg = *cfg;
     return TSP_OK;
 }
@@ -591,8 +643,8 @@ bool cache_lookup(const std::string &key, tsp_kernel_cfg *cfg) {
         char k[256];
         tsp_kernel_cfg c{};
         double ms = 0;
-        if (std::sscanf(line, "%255s %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
-                        &c.zalt, &c.halo_ring, &ms) == 8 &&
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
+                        &c.zalt, &c.halo_ring, &c.stage_pairs, &ms) == 9 &&
             key == k && validate_cfg(&c) == TSP_OK) {
             *cfg = c; // the last matching line wins
             found = true;
@@ -607,8 +659,8 @@ void cache_store(const std::string &key, const tsp_kernel_cfg &c, double ms) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap, c.zalt,
-                 c.halo_ring, ms);
+    std::fprintf(f, "%s %d %d %d %d %d %d %d %.6f\n", key.c_str(), c.ghost_mode, c.tile_y, c.zchunk, c.xcd_remap,
+                 c.zalt, c.halo_ring, c.stage_pairs, ms);
     std::fclose(f);
 }
 
@@ -657,7 +709,11 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
         for (int zc : zchunks)
             for (int xr = 0; xr < 2; ++xr)
                 for (int za = 0; za < 2; ++za)
-                    for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr});
+                    for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr, 0});
+    // stage pairs (32 x 32 tiles; tile_y, zalt and halo_ring do not apply)
+    if (pairs_apply(s, 0, nullptr))
+        for (int zc : zchunks)
+            for (int xr = 0; xr < 2; ++xr) cands.push_back(tsp_kernel_cfg{0, s->cfg.tile_y, zc, xr, 0, 0, 1});
     double best_ms = 1e300;
     auto time_cfg = [&](const tsp_kernel_cfg &c) -> tsp_status {
         s->cfg = c;
@@ -685,6 +741,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     if (st == TSP_OK) {
         tsp_kernel_cfg fill = best;
         fill.ghost_mode = 1;
+        fill.stage_pairs = 0;
         st = time_cfg(fill);
     }
     // restore
@@ -741,7 +798,7 @@ struct tsp_adjoint {
     tsp::LaunchFn launch = nullptr;
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
-    tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1, -1};
+    tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1, -1, -1};
 };
 
 namespace {

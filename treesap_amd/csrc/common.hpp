@@ -125,6 +125,7 @@ struct StageArgs {
     double b, a_next;        // dt*b_s and dt*a_{s+1} (integrators/SingleLevelRK4Integrator.hpp:119-121)
     double c_init_argn;      // adjoint only: coefficient of lambda' in the next stage argument
     double src_scale;        // adjoint only: factor applied to every source value on load
+    double b1, a_next1;      // stage-pair kernel: dt*b and dt*a_next of the pair's second stage
     double gf1[3];           // 1/dx   (stencil_impl.hpp:213, stencil.hpp:94)
     double gf2[3];           // 1/(dx*dx) (stencil_impl.hpp:255)
     double ko_strength;
@@ -141,6 +142,10 @@ LaunchFn lookup_phipi_adjoint(int n, int r);
 LaunchFn lookup_phiu(int n, int r);
 LaunchFn lookup_nonlinear(int n, int r);
 LaunchFn lookup_tensor(int n, int r);
+// Launch a stage-pair kernel (forward RK4 stages first, first+1 in one launch;
+// first = 0 or 2); hipErrorNotSupported, without launching, where it does not apply.
+using PairFn = hipError_t (*)(const StageArgs &a, int first, hipStream_t stream);
+PairFn lookup_pair_phipi(int n, int r);
 // launchers specialised for forward RK4 stage 0 / stage 3 (hipErrorNotSupported
 // for configurations they do not cover: the caller then uses the generic one)
 LaunchFn lookup_phipi_s0(int n, int r);

@@ -1,0 +1,21 @@
+// Stage-pair kernels of the (phi, Pi) wave for every (stencil width, KO order)
+// with KO dissipation (the pair kernel reads every component with stencils).
+#include "pair_kernel.hpp"
+
+namespace tsp {
+
+PairFn lookup_pair_phipi(int n, int r) {
+#define TSP_CASE(NN, RR)                                                                                              \
+    if (n == NN && r == RR) return &launch_pair<SysPhiPi, NN, RR>;
+    TSP_CASE(1, 1)
+    TSP_CASE(1, 2)
+    TSP_CASE(2, 2)
+    TSP_CASE(2, 3)
+    TSP_CASE(3, 3)
+    TSP_CASE(3, 4)
+    TSP_CASE(4, 4)
+#undef TSP_CASE
+    return nullptr;
+}
+
+} // namespace tsp
