@@ -315,13 +315,17 @@ def main() -> int:
     s.close()
 
     value = cells * args.steps / wall
-    mean_launch_s = dev_ms / 1e3 / (4 * args.steps)
-    bytes_per_launch = bytes_per_cell * cells / 4
+    # four stage launches per step, or two with stage pairs (the algorithmic bytes of a pair
+    # launch are those of its two stages)
+    pairs = bool(cfg.get("stage_pairs"))
+    launches = 2 if pairs else 4
+    mean_launch_s = dev_ms / 1e3 / (launches * args.steps)
+    bytes_per_launch = bytes_per_cell * cells / launches
     achieved = bytes_per_launch / mean_launch_s / 1e9
-    stage_gbs = [
-        (b * case.ncomp * cells) / (ms / 1e3 / 10) / 1e9
-        for b, ms in zip((24, 40, 40, 24), stage_ms)
-    ]
+    # per launch kind: stages 0..3, or pairs (0,1) and (2,3) (profile_rk4 reports a pair in
+    # the slot of its first stage)
+    kinds = ((64, stage_ms[0]), (64, stage_ms[2])) if pairs else tuple(zip((24, 40, 40, 24), stage_ms))
+    stage_gbs = [(b * case.ncomp * cells) / (ms / 1e3 / 10) / 1e9 for b, ms in kinds]
     log(f"[bench] {args.steps} steps: wall {wall * 1e3:.1f} ms, device {dev_ms:.1f} ms, "
         f"{value:.3e} cell-updates/s, per-stage GB/s {[round(x) for x in stage_gbs]}")
 
@@ -386,6 +390,7 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(cfg),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "launches_per_step": launches,
             "mean_stage_launch_us": mean_launch_s * 1e6,
             "stage_gbs_eager": stage_gbs,
         },
