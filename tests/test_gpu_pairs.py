@@ -120,3 +120,34 @@ def test_wave256_full_size_one_step_with_pairs_matches_oracle():
     got, _, _ = _solve(case, state, 1, dict(ghost_mode=0, zchunk=64, xcd_remap=1, stage_pairs=1))
     want, _, _ = _oracle(case, state, 1)
     assert np.array_equal(got, want), f"max abs diff {np.max(np.abs(got - want))}"
+
+
+def test_tune_cache_round_trip_with_pairs(monkeypatch, tmp_path):
+    """The tuning cache stores all seven configuration fields; a stored stage-pair configuration is
+    taken as is (last matching line wins) and stays bitwise, and lines of the six-field format are
+    ignored."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("TSP_TUNE_CACHE", str(cache))
+    case, state = small_case("phipi", n=(48, 40, 24), stencil_n=2, ko_order=2, periodic=(1, 1, 1), seed=5)
+    s = Solver(case)
+    s.set_state(state)
+    first = s.autotune(1)
+    s.close()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and len(lines[0].split()) == 9
+    key = lines[0].split()[0]
+    assert [int(v) for v in lines[0].split()[1:8]] == [first[k] for k in
+                                                        ("ghost_mode", "tile_y", "zchunk", "xcd_remap", "zalt",
+                                                         "halo_ring", "stage_pairs")]
+    with cache.open("a") as f:
+        f.write(f"{key} 0 8 12 0 0 0 1 0.5\n")   # a stage-pair configuration
+        f.write(f"{key} 0 16 64 1 1 1 0.4\n")    # six-field format: ignored
+    s = Solver(case)
+    s.set_state(state)
+    cfg = s.autotune(1)
+    assert cfg == dict(ghost_mode=0, tile_y=8, zchunk=12, xcd_remap=0, zalt=0, halo_ring=0, stage_pairs=1)
+    s.rk4_steps(3)
+    got = s.get_state()
+    s.close()
+    want, _, _ = _oracle(case, state, 3)
+    assert np.array_equal(got, want)

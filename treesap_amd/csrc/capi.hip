@@ -632,7 +632,22 @@ std::string tune_key(const tsp_solver *s) {
     return buf;
 }
 
-bool cache_lookup(const std::string &key, tsp_kernel_cfg *cfg) {
+// Number of whitespace-separated fields of a cache line.
+int count_fields(const char *line) {
+    int n = 0;
+    bool in = false;
+    for (const char *p = line; *p; ++p) {
+        const bool sp = *p == ' ' || *p == '\t' || *p == '\n' || *p == '\r';
+        if (!sp && !in) ++n;
+        in = !sp;
+    }
+    return n;
+}
+
+// A cached configuration for `key` that this solver can run (the last matching line wins).
+// Lines are "key ghost_mode tile_y zchunk xcd_remap zalt halo_ring stage_pairs ms"; lines with
+// another field count (the format before stage_pairs) are skipped.
+bool cache_lookup(const tsp_solver *s, const std::string &key, tsp_kernel_cfg *cfg) {
     const char *path = std::getenv("TSP_TUNE_CACHE");
     if (!path || !*path) return false;
     FILE *f = std::fopen(path, "r");
@@ -643,10 +658,11 @@ bool cache_lookup(const std::string &key, tsp_kernel_cfg *cfg) {
         char k[256];
         tsp_kernel_cfg c{};
         double ms = 0;
-        if (std::sscanf(line, "%255s %d %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
+        if (count_fields(line) == 9 &&
+            std::sscanf(line, "%255s %d %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
                         &c.zalt, &c.halo_ring, &c.stage_pairs, &ms) == 9 &&
-            key == k && validate_cfg(&c) == TSP_OK) {
-            *cfg = c; // the last matching line wins
+            key == k && validate_cfg(&c) == TSP_OK && (!c.stage_pairs || pairs_apply(s, c.ghost_mode, nullptr))) {
+            *cfg = c;
             found = true;
         }
     }
@@ -675,7 +691,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
         if (chosen) *chosen = best;
         return TSP_OK;
     }
-    if (cache_lookup(key, &best)) {
+    if (cache_lookup(s, key, &best)) {
         s->cfg = best;
         if (chosen) *chosen = best;
         return TSP_OK;
