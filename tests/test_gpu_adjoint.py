@@ -197,3 +197,22 @@ def test_autograd_function_on_a_reduced_adjoint_workload():
     fd = float((lp - lm) / (2 * eps))
     assert float((x.grad * v).sum()) == pytest.approx(fd, rel=1e-7)
     prob.close()
+
+
+@pytest.mark.parametrize("ko,per,abl", [(3, (1, 1, 1), 0), (2, (1, 0, 1), 3)])
+def test_backward_with_stage_pairs_matches_oracle(ko, per, abl):
+    """Forward sweep with the stage-pair kernels: the adjoint reads only the forward's state_old
+    (receiver trace and checkpoints), so trace and gradient stay bitwise, in both backward modes."""
+    _say(f"[adjoint] stage pairs ko={ko} periodic={per} abl={abl}")
+    case, st, rec, target, nsteps, K = _problem(ko=ko, per=per, abl=abl)
+    s = Solver(case)
+    s.kernel = dict(ghost_mode=0, stage_pairs=1)
+    s.set_state(st)
+    a = Adjoint(s, rec, nsteps, K)
+    a.forward()
+    trace = a.trace()
+    otrace, _, ograd = _oracle_trace_and_grad(case, st, rec, target, nsteps)
+    assert np.array_equal(trace, otrace)
+    dtrace = 2.0 * (trace - target)
+    assert np.array_equal(a.backward(dtrace), ograd)
+    assert np.array_equal(a.backward(dtrace, recompute=True), ograd)

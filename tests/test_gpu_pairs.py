@@ -151,3 +151,18 @@ def test_tune_cache_round_trip_with_pairs(monkeypatch, tmp_path):
     s.close()
     want, _, _ = _oracle(case, state, 3)
     assert np.array_equal(got, want)
+
+
+def test_euler_steps_unaffected_by_pair_mode():
+    """Euler steps run the single-stage kernel whatever stage_pairs says."""
+    case, state = small_case("phipi", n=(30, 20, 18), stencil_n=2, ko_order=3, periodic=(1, 0, 1), abl=3, seed=13)
+    s = Solver(case)
+    s.kernel = dict(ghost_mode=0, stage_pairs=1)
+    s.set_state(state)
+    s.euler_steps(3)
+    got = s.get_state()
+    s.close()
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    o.euler_steps(3)
+    assert np.array_equal(got, o.get_state())
