@@ -5,7 +5,8 @@
 // lockstep, so the register roles (state_old, state_new, arg, arg_next) rotate
 // identically for all of them and travel as kernel arguments. The work list
 // holds (instance, tile, z chunk) items, largest instances first; a grid of
-// `gridDim.x` persistent blocks strides over it. Each item runs exactly the
+// `gridDim.x` persistent blocks strides over it, dealing runs of consecutive
+// items to one XCD (xcd_item). Each item runs exactly the
 // tile body of the single-box kernel (FusedStage::run_tile), so a batched
 // solve is bit-identical to solving every instance on its own.
 #pragma once
@@ -53,6 +54,20 @@ inline int batch_variant(int n, int r) {
     return -1;
 }
 
+// Work item of block slot w. The hardware deals blocks to the 8 XCDs round
+// robin (slot w runs on XCD w mod 8); items go to the XCDs in runs of
+// XCD_RUN consecutive items, so neighbouring tiles of one z chunk of an
+// instance share one XCD's L2 (their halo rows are each other's queue heads)
+// while every XCD still gets a share of the large and the small instances.
+// A permutation of [0, nwork); slots past the last full round keep their item.
+constexpr int XCD_RUN = 16;
+__device__ __forceinline__ int xcd_item(int w, int nwork) {
+    const int full = nwork / (8 * XCD_RUN) * (8 * XCD_RUN);
+    if (w >= full) return w;
+    const int x = w % 8, r = w / 8; // XCD of the slot and its rank among that XCD's slots
+    return (r / XCD_RUN) * (8 * XCD_RUN) + x * XCD_RUN + r % XCD_RUN;
+}
+
 // One (stencil width, KO order) group: a persistent grid strides over the
 // group's work items. RS > 0: the halo-ring tile body. Items are 64 x TY
 // tiles or, flagged narrow, 32 x 2*TY tiles (same thread count).
@@ -67,7 +82,7 @@ template <class Sys, int N, int R, int TY, int RS, int SM = -1> struct BatchedSt
                                                const BatchWork *__restrict__ work, const BatchArgs &b, double *lds,
                                                double *sig_lds) {
         for (int w = blockIdx.x; w < b.nwork; w += gridDim.x) {
-            const BatchWork wk = work[w];
+            const BatchWork wk = work[xcd_item(w, b.nwork)];
             const BatchInstance &I = inst[wk.inst];
             StageArgs a;
             a.src = I.reg[b.src];
