@@ -48,10 +48,10 @@ template <class Sys, int N, int R, int P> struct StagePair {
     static_assert(Staging<Sys, R>::count() == NC, "every component is read with stencils (KO order > 0)");
     static constexpr int TE = 32;         // extended tile edge, one cell per thread and row
     static constexpr int TI = TE - 2 * H; // output tile edge
-    // rows per thread: two queues of 2H+1 planes per cell need more than the
-    // 128 VGPRs of a 1024-thread block from H = 3 on; 512 threads with two
-    // rows each (rows r and r + 16) have 256
-    static constexpr int RPT = H >= 3 ? 2 : 1;
+    // rows per thread: up to H = 3 the two queues of 2H+1 planes per cell fit
+    // the 128 VGPRs of a 1024-thread block; at H = 4 they spill there, and
+    // 512 threads with two rows each (rows r and r + 16) have 256 VGPRs
+    static constexpr int RPT = H >= 4 ? 2 : 1;
     static constexpr int NT = TE * TE / RPT;
     static constexpr int RSTEP = TE / RPT; // row distance of a thread's rows
     static constexpr int LA = TE + 2 * H;  // source plane edge with its halo
