@@ -1,5 +1,5 @@
 // SingleLevelIntegrator.hpp -- integrator on one uniform level whose state
-// lives in device memory behind the C ABI (SingleLevelIntegrator.hpp:36-193).
+// lives in device memory behind the C ABI (SingleLevelIntegrator.hpp:22-187).
 //
 // Construction maps the model onto a device solver (device/model_map.hpp) and
 // allocates its registers; `block_size` is accepted for source compatibility.

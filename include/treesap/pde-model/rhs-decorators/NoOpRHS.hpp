@@ -1,5 +1,5 @@
 // NoOpRHS.hpp -- right-hand side that is zero for every component
-// (pde-model/rhs-decorators/NoOpRHS.hpp:19-41); used in concept checks.
+// (pde-model/rhs-decorators/NoOpRHS.hpp:19-33); used in concept checks.
 #pragma once
 
 #include "../PDEModelConcepts.hpp"

@@ -1,6 +1,6 @@
 // PlotExporter.hpp -- writes the state every `export_interval` steps as a
 // single-level, single-box plotfile named path + step (zero-padded to 5
-// digits), like util/PlotExporter.hpp:31-77 does through
+// digits), like util/PlotExporter.hpp:29-67 does through
 // amrex::WriteSingleLevelPlotfile. The layout follows AMReX's documented
 // plotfile format (HyperCLaw-V1.1 text header, a Level_0/Cell_H FabArray
 // header and one binary FAB file with a native-order IEEE double descriptor);

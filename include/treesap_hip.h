@@ -60,7 +60,7 @@ typedef enum tsp_bc { TSP_BC_NOOP = 0, TSP_BC_DIRICHLET = 1 } tsp_bc;
 /* ABLProfileConcept (rhs-decorators/ABLDecorator.hpp:24-29) instances. */
 typedef enum tsp_profile { TSP_PROFILE_LINEAR = 0, TSP_PROFILE_QUADRATIC = 1, TSP_PROFILE_CUBIC = 2 } tsp_profile;
 
-/* Register roles (SingleLevelIntegrator.hpp:332-333 and
+/* Register roles (SingleLevelIntegrator.hpp:182-183 and
  * integrators/SingleLevelRK4Integrator.hpp:162-166). */
 typedef enum tsp_role { TSP_ROLE_OLD = 0, TSP_ROLE_NEW = 1, TSP_ROLE_ARG = 2, TSP_ROLE_ARG_NEXT = 3 } tsp_role;
 
@@ -118,7 +118,7 @@ const char *tsp_version(void);
 tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n);
 
 /* Replaces SingleLevelRK4Integrator(model, block_size)
- * (integrators/SingleLevelRK4Integrator.hpp:65-67, SingleLevelIntegrator.hpp:197-199):
+ * (integrators/SingleLevelRK4Integrator.hpp:65-67, SingleLevelIntegrator.hpp:47-49):
  * validates the model, allocates the four registers (state_old, state_new,
  * arg, arg_next) in device memory. `stream` is a hipStream_t or NULL (the
  * library then creates its own default-priority stream). The descriptor is

@@ -139,7 +139,7 @@ typedef struct {
     int64_t gx, gy, gz, gcells;
     double dx[3];
     double *reg[4];
-    /* register roles (SingleLevelIntegrator.hpp:332-333,
+    /* register roles (SingleLevelIntegrator.hpp:182-183,
        integrators/SingleLevelRK4Integrator.hpp:162-166) */
     int r_old, r_new, r_arg, r_argn;
     double time;

@@ -185,6 +185,54 @@ def test_dropin_program_matches_oracle(gpu_runs, scenario):
     assert np.isfinite(final).all() and np.abs(final).max() > 0
 
 
+# The initial data of each program (tests/cpp/dropin_gpu.cpp), restated with
+# Python's math: (box lo, box hi, cells per axis, one function per component).
+def _tensor_init(c):
+    a, b, e = c + 1, (c % 3) + 1, (c % 4) + 1
+    return lambda x, y, z: 0.01 * math.sin(2 * math.pi * (a * x + b * y)) * math.cos(2 * math.pi * e * z)
+
+
+INIT_EXPRS = {
+    "first_order": ((-2.0,) * 3, (2.0,) * 3, (32, 32, 32),
+                    [lambda x, y, z: math.exp(-(x ** 2 + y ** 2 + z ** 2) / 0.5 ** 2)] + [lambda x, y, z: 0.0] * 3),
+    "abl_ko": ((0.0,) * 3, (1.0,) * 3, (48, 48, 48),
+               [lambda x, y, z: 0.8 * math.exp(-((x - 0.45) ** 2 + (y - 0.55) ** 2 + (z - 0.5) ** 2) / 0.1 ** 2)
+                - 0.5 * math.exp(-((x - 0.6) ** 2 + (y - 0.4) ** 2 + (z - 0.55) ** 2) / 0.07 ** 2),
+                lambda x, y, z: 0.0]),
+    "nonlinear_euler": ((0.0,) * 3, (1.0, 0.9, 0.8), (40, 36, 32),
+                        [lambda x, y, z: math.sin(2 * math.pi * x) * math.cos(2 * math.pi * y / 0.9)
+                         + 0.3 * math.sin(2 * math.pi * z / 0.8),
+                         lambda x, y, z: 0.5 * math.cos(2 * math.pi * (x + y))]),
+    "tensor": ((0.0,) * 3, (1.0,) * 3, (24, 24, 24), [_tensor_init(c) for c in range(12)]),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", SCENARIOS)
+def test_dropin_initial_state_matches_cell_centres(gpu_runs, scenario):
+    """initialise() evaluates the InitConcept functor at the cell centres
+    x = lo + (i + 0.5) * dx with dx = (hi - lo) / n (SingleLevelIntegrator.hpp:75-93,
+    Geometry::CellSize): the uploaded state equals Python's evaluation of the
+    same expressions there, bit for bit. A half-cell shift, a dx from n - 1 or
+    a swapped axis fails it."""
+    d = gpu_runs / scenario
+    case = case_from_desc(_read_desc(d / "desc.bin"))
+    lo, hi, n, fns = INIT_EXPRS[scenario]
+    assert tuple(case.n) == n and tuple(case.prob_lo) == lo and tuple(case.prob_hi) == hi
+    init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
+    dx = [(hi[a] - lo[a]) / n[a] for a in range(3)]
+    xs = [[lo[a] + (i + 0.5) * dx[a] for i in range(n[a])] for a in range(3)]
+    want = np.empty(case.shape)
+    for c, fn in enumerate(fns):
+        for k, z in enumerate(xs[2]):
+            for j, y in enumerate(xs[1]):
+                want[c, k, j] = [fn(x, y, z) for x in xs[0]]
+    assert np.array_equal(init, want), f"max abs diff {np.max(np.abs(init - want))}"
+    # the check has teeth: a half-cell shift of x changes the state
+    shifted = [lo[0] + i * dx[0] for i in range(n[0])]
+    assert not np.array_equal(init[0, 0, 0], [fns[0](x, xs[1][0], xs[2][0]) for x in shifted])
+
+
 @pytest.mark.gpu
 def test_dropin_plotfiles(gpu_runs):
     """configureExporter(path, 4) writes the state before steps 0, 4 and 8 of 12."""

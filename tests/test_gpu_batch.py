@@ -84,6 +84,30 @@ def test_batch_graph_equals_eager():
         assert np.array_equal(a.get_state(i), g.get_state(i))
 
 
+def test_batch_graph_after_config_switch():
+    """A captured graph holds the work list of its configuration: graph steps
+    under A, a switch to B (work list rebuilt), eager steps, back to A (rebuilt
+    again), graph steps: every instance equals plain eager stepping."""
+    _say("[batch] graph after config switch")
+    cases = mixed_cases()
+    A = dict(BATCH_CFGS[0])
+    B = dict(BATCH_CFGS[1])
+    a = Batch([c for c, _ in cases])
+    g = Batch([c for c, _ in cases])
+    for i, (_, st) in enumerate(cases):
+        a.set_state(i, st)
+        g.set_state(i, st)
+    a.rk4_steps(9)
+    g.kernel = A
+    g.graph_rk4_steps(4)
+    g.kernel = B
+    g.rk4_steps(1)
+    g.kernel = A
+    g.graph_rk4_steps(4)
+    for i in range(len(cases)):
+        assert np.array_equal(a.get_state(i), g.get_state(i)), i
+
+
 def test_small_ensemble_workload():
     """A 12-instance slice of the ensemble workload (sides 32..64, n in {2,3,4}) for 20 steps."""
     _say("[batch] ensemble slice")

@@ -182,6 +182,107 @@ def test_abl_sigma_brute_force(profile):
         assert s.abl_sigma(i, j, k) == pytest.approx(want, rel=1e-15, abs=0)
 
 
+def _sigma_brute(n, T, smax, profile):
+    """sigma(i,j,k) of ABLDecorator.hpp:115-132 by brute force: distance to the nearest bulk cell."""
+    p = {"linear": lambda v: v, "quadratic": lambda v: v * v, "cubic": lambda v: v ** 3}[profile]
+    lo = np.array([T, T, T])
+    hi = np.array([n[0] - 1 - T, n[1] - 1 - T, n[2] - 1 - T])
+    sig = np.zeros((n[2], n[1], n[0]))
+    for k, j, i in itertools.product(range(n[2]), range(n[1]), range(n[0])):
+        iv = np.array([i, j, k])
+        d = float(np.sqrt(((iv - np.clip(iv, lo, hi)) ** 2).sum()))
+        sig[k, j, i] = 0.0 if d == 0 else smax * p(min(d / T, 1.0))
+    return sig
+
+
+def _rk4_affine_step(s, t0, t1, phi, pi, dt):
+    """One exact RK4 step of the spatially uniform ODE phi' = pi - s (phi - t0), pi' = -s (pi - t1).
+
+    RK4 on a linear autonomous system is the degree-4 Taylor polynomial of the
+    propagator; the affine forcing enters through the augmented matrix
+    M = [[-s, 1, s t0], [0, -s, s t1], [0, 0, 0]] acting on (phi, pi, 1)."""
+    M = np.array([[-s, 1.0, s * t0], [0.0, -s, s * t1], [0.0, 0.0, 0.0]]) * dt
+    P = np.eye(3) + M + M @ M / 2 + M @ M @ M / 6 + M @ M @ M @ M / 24
+    z = P @ np.array([phi, pi, 1.0])
+    return z[0], z[1]
+
+
+ABL_CLOSED_CASES = [
+    # stencil half-width, KO order, box, layer thickness, max rate, profile, targets (phi, Pi)
+    (1, 0, (22, 21, 23), 6, 2.0, "quadratic", (0.25, -0.375)),
+    (2, 3, (50, 49, 51), 12, 1.5, "linear", (-0.5, 0.125)),
+]
+
+
+@pytest.mark.parametrize("n_st,ko,n,T,smax,profile,tgt", ABL_CLOSED_CASES)
+def test_abl_damping_closed_form(n_st, ko, n, T, smax, profile, tgt):
+    """ABLDecorator.hpp:102-108: F<c> - sigma(iv) (u_c - target[c]).
+
+    A spatially constant (phi, Pi) = (a, p) on a periodic box stays constant
+    wherever sigma is constant over the cells one RK4 step reads (four stages
+    of axis stencils of half-width h = max(n, r): offsets d with
+    sum_a ceil(|d_a| / h) <= 4); there the stencil terms vanish and the step
+    is the RK4 polynomial of the damped affine ODE. Cells in the bulk
+    (sigma = 0) and in the saturated corners (sigma = max) are checked against
+    that closed form."""
+    case = _case(n=n, stencil_n=n_st, ghost=max(n_st, ko), ko_order=ko, ko_strength=0.1 if ko else 0.0,
+                 abl_thickness=T, abl_max=smax, abl_profile=profile, target=tgt + (0.0,) * 10, dt=0.25 / n[0])
+    a, p = 0.75, 0.5
+    s = orc.OracleSolver(case)
+    st = np.zeros(s.shape)
+    st[0] = a
+    st[1] = p
+    s.set_state(st)
+    s.rk4_steps(1)
+    got = s.get_state()
+    sig = _sigma_brute(n, T, smax, profile)
+    h = max(n_st, ko)
+    rad = 4 * h
+    reach = [(dx, dy, dz) for dz in range(-rad, rad + 1) for dy in range(-rad, rad + 1) for dx in range(-rad, rad + 1)
+             if sum(-(-abs(v) // h) for v in (dx, dy, dz)) <= 4]
+    reach = np.array(reach)
+    checked = {0.0: 0, smax: 0}
+    for k, j, i in itertools.product(range(n[2]), range(n[1]), range(n[0])):
+        sv = sig[k, j, i]
+        if sv not in checked:
+            continue
+        ii = (i + reach[:, 0]) % n[0]
+        jj = (j + reach[:, 1]) % n[1]
+        kk = (k + reach[:, 2]) % n[2]
+        if not np.all(sig[kk, jj, ii] == sv):
+            continue
+        want = _rk4_affine_step(sv, tgt[0], tgt[1], a, p, case["dt"])
+        assert got[0, k, j, i] == pytest.approx(want[0], rel=1e-13, abs=1e-14), (i, j, k, sv)
+        assert got[1, k, j, i] == pytest.approx(want[1], rel=1e-13, abs=1e-14), (i, j, k, sv)
+        checked[sv] += 1
+    assert checked[0.0] > 0 and checked[smax] > 0, checked
+
+
+def test_abl_damping_closed_form_is_sensitive():
+    """The closed form separates the reference's damping from plausible slips:
+    a flipped sign, swapped targets, or a damping of phi only."""
+    n_st, ko, n, T, smax, profile, tgt = ABL_CLOSED_CASES[0]
+    case = _case(n=n, stencil_n=n_st, ghost=n_st, abl_thickness=T, abl_max=smax, abl_profile=profile,
+                 target=tgt + (0.0,) * 10, dt=0.25 / n[0])
+    a, p = 0.75, 0.5
+    s = orc.OracleSolver(case)
+    st = np.zeros(s.shape)
+    st[0] = a
+    st[1] = p
+    s.set_state(st)
+    s.rk4_steps(1)
+    corner = s.get_state()[:, 0, 0, 0]
+    dt = case["dt"]
+    right = _rk4_affine_step(smax, tgt[0], tgt[1], a, p, dt)
+    assert corner[0] == pytest.approx(right[0], rel=1e-13) and corner[1] == pytest.approx(right[1], rel=1e-13)
+    flipped = _rk4_affine_step(-smax, tgt[0], tgt[1], a, p, dt)
+    swapped = _rk4_affine_step(smax, tgt[1], tgt[0], a, p, dt)
+    M = np.array([[-smax, 1.0, smax * tgt[0]], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]) * dt
+    phi_only = (np.eye(3) + M + M @ M / 2 + M @ M @ M / 6 + M @ M @ M @ M / 24) @ np.array([a, p, 1.0])
+    for wrong in (flipped, swapped, phi_only[:2]):
+        assert max(abs(corner[0] - wrong[0]), abs(corner[1] - wrong[1])) > 1e-4
+
+
 @pytest.mark.parametrize("periodic,bc", [((1, 1, 1), "noop"), ((0, 0, 0), "dirichlet"), ((1, 0, 1), "dirichlet"),
                                          ((0, 1, 0), "noop")])
 def test_ghost_fill_brute_force(periodic, bc):

@@ -75,6 +75,15 @@ tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
 
 int auto_zchunk(int nz) { return std::min(nz, 32); }
 
+void drop_batch_graphs(tsp_batch *b) {
+    for (auto &g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    b->graph_cfg = tsp_batch_cfg{-1, -1, -1, -1, -1, -1};
+}
+
 // (Re)build the device work list: instances in decreasing cell count, each
 // split into 64 x tile_y column tiles and z chunks; with narrow_tiles, an x
 // remainder of at most 32 cells becomes a strip of 32 x 2*tile_y tiles.
@@ -110,7 +119,9 @@ tsp_status build_work(tsp_batch *b) {
     }
     if (work.size() > 0x7fffffff) return fail(TSP_UNSUPPORTED, "work list too long");
     if (b->d_work) {
+        // captured graphs hold pointers into the old work list
         TSP_HIP(hipStreamSynchronize(b->stream));
+        drop_batch_graphs(b);
         (void)hipFree(b->d_work);
         b->d_work = nullptr;
     }
@@ -120,14 +131,6 @@ tsp_status build_work(tsp_batch *b) {
     b->nwork = (int)work.size();
     b->work_cfg = b->cfg;
     return TSP_OK;
-}
-
-void drop_batch_graphs(tsp_batch *b) {
-    for (auto &g : b->graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
 }
 
 // One RK4 step of every instance: per stage, one persistent launch per

@@ -3,7 +3,7 @@
 // derivatives plus Kreiss-Oliger dissipation).
 //
 // What one launch computes is integrators/SingleLevelRK4Integrator.hpp:137-156
-// (or SingleLevelEulerIntegrator.hpp:245-251) over the whole box: per valid
+// (or SingleLevelEulerIntegrator.hpp:66-88) over the whole box: per valid
 // cell, components in order 0..ncomp-1, k = F<c>(src) with F the decorated RHS
 // ABLDecorator<KODecorator<User>> (ABLDecorator.hpp:102-108,
 // KODecorator.hpp:58-70), then the stage axpys.
