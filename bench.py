@@ -53,21 +53,41 @@ def traffic_from_profiles(cfg: dict):
         return None
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, state, budget_s: float):
+    """The C oracle on this host: RK4 steps of the same workload with all host threads for about
+    `budget_s` seconds, then one step with one thread."""
     from oracle import oracle as orc
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    o = orc.OracleSolver(case)
-    o.set_state(state)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        o.rk4_steps(1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or el / steps * (steps + 1) > 2.0 * budget_s:
-            break
-    o.close()
+
+    def run(nthreads, budget, max_steps):
+        orc.set_threads(nthreads)
+        o = orc.OracleSolver(case)
+        o.set_state(state)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            o.rk4_steps(1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if steps >= max_steps or el >= budget or el / steps * (steps + 1) > 2.0 * budget:
+                break
+        o.close()
+        return steps, el
+
+    steps, el = run(threads, budget_s, 1 << 30)
+    steps1, el1 = run(1, 0.0, 1)
+    orc.set_threads(threads)
     return {
         "value": case.cells * steps / el,
         "unit": "cell-updates/s",
@@ -75,6 +95,9 @@ def cpu_baseline(case, state, budget_s: float):
         "kind": "port",
         "sample": f"{steps} RK4 step(s) of the same 256^3 workload ({case.cells} cells) by the C oracle, "
                   f"OpenMP {threads} threads, {el:.1f} s",
+        "cpu_model": _cpu_model(),
+        "single_thread_value": case.cells * steps1 / el1,
+        "single_thread_sample": f"{steps1} RK4 step of the same workload on 1 thread, {el1:.1f} s",
     }
 
 
@@ -109,19 +132,28 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     ranked = sorted((timed(cfg, 6), i) for i, cfg in enumerate(cfgs))
     best = min((timed(cfgs[i], 20), dict(cfgs[i])) for _, i in ranked[:3])
     b.kernel = best[1]
+    b.graph_rk4_steps(2)  # capture the graph of the chosen configuration (an even count keeps the parity)
     for i, (case, centre) in enumerate(params):
         b.set_state(i, gaussian_state_device(case, centre))
-    b.graph_rk4_steps(2)
     batched_ms = b.time_rk4(steps, use_graph=True)
-    b.close()
     torch.cuda.synchronize()
 
+    # plain path: the same instances one after another; every instance must end
+    # bitwise equal to its batched solve
     plain_ms = 0.0
-    for case, centre in params:
+    mismatched = 0
+    for i, (case, centre) in enumerate(params):
         s = Solver(case)
         s.set_state(gaussian_state_device(case, centre))
         plain_ms += s.time_rk4(steps, use_graph=False)
+        got = s.get_state(out=torch.empty(case.shape, dtype=torch.float64, device="cuda"))
+        want = b.get_state(i, out=torch.empty(case.shape, dtype=torch.float64, device="cuda"))
+        if not bool(torch.isfinite(want).all()) or not torch.equal(got, want):
+            mismatched += 1
         s.close()
+    b.close()
+    if mismatched:
+        raise RuntimeError(f"ensemble: {mismatched} instance(s) differ between the batched and plain solves")
     batched = cells * steps / (batched_ms / 1e3)
     plain = cells * steps / (plain_ms / 1e3)
     return {
@@ -136,6 +168,7 @@ def ensemble_leg(count: int, steps: int, tune: bool):
         "batched_s": batched_ms / 1e3,
         "plain_s": plain_ms / 1e3,
         "batch_kernel": best[1],
+        "batched_equals_plain": "bitwise, every instance",
     }
 
 
@@ -153,6 +186,9 @@ def tensor_leg(n: int, steps: int):
     s.graph_rk4_steps(2)
     ms = s.time_rk4(steps, use_graph=True)
     stage_ms, fill_ms = s.profile_rk4(2)
+    st = s.get_state(out=torch.empty(case.shape, dtype=torch.float64, device="cuda"))
+    if not bool(torch.isfinite(st).all()):
+        raise RuntimeError("tensor leg: non-finite state after the timed steps")
     s.close()
     cu = case.cells * steps / (ms / 1e3)
     return {
@@ -179,6 +215,7 @@ def adjoint_leg(n: int, batch: int, nsteps: int, K: int):
     64 receivers; loss and gradient through the torch.autograd.Function (forward with receiver trace and
     checkpoints every K steps, backward through the transposed RK4 stage kernels), against the plain
     forward solve of the same sources (graph-replayed steps)."""
+    import numpy as np
     import torch
 
     from treesap_amd import Solver
@@ -216,6 +253,8 @@ def adjoint_leg(n: int, batch: int, nsteps: int, K: int):
         out["recompute" if recompute else "stored_trace"] = {
             "forward_s": t1 - t0, "backward_s": t2 - t1, "adjoint_over_forward": (t2 - t0) / t_plain}
         gnorm = float(x.grad.norm())
+        if not (np.isfinite(float(loss.detach())) and bool(torch.isfinite(x.grad).all())):
+            raise RuntimeError("adjoint leg: non-finite loss or gradient")
     prob.close()
     return {
         "workload": f"adjoint{n}: (phi,Pi) {n}^3, n=2, no KO, 12-cell ABL, {batch} sources x {nsteps} RK4 steps, "
@@ -274,6 +313,9 @@ def main() -> int:
     cfg = s.kernel
 
     s.graph_rk4_steps(args.warmup)
+    # graphs of both step parities exist before the timed region (an odd warmup
+    # count would otherwise capture one inside it)
+    s.graph_prepare()
     s.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -167,9 +167,14 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen);
 
 /* hipGraph replay of RK4 steps (no reference counterpart). The register
  * rotation of the reference has a period of two steps, so one graph holds two
- * steps; tsp_graph_rk4_steps replays it and runs an odd remainder eagerly.
- * Results are bit-identical to tsp_rk4_steps. */
+ * steps (captured on first use for the current register roles);
+ * tsp_graph_rk4_steps replays it and runs an odd remainder eagerly. Results
+ * are bit-identical to tsp_rk4_steps. */
 tsp_status tsp_graph_rk4_steps(tsp_solver *s, int64_t nsteps);
+/* Capture and instantiate, without launching, the two-step graphs for the
+ * current register roles and for those one RK4 step later, so that a later
+ * tsp_graph_rk4_steps starting at either step parity only replays. */
+tsp_status tsp_graph_prepare(tsp_solver *s);
 
 /* Run `nsteps` eager RK4 steps recording HIP events around every stage
  * launch on the solver's stream; writes the summed device time of each stage

@@ -18,10 +18,13 @@
  *   stencils : polynomial exactness, Fourier symbols               (pinned)
  *   RK4/Euler: single-Fourier-mode closed form G(k)^N              (pinned)
  *   ABL sigma: brute-force distance formula on tiny grids          (pinned)
+ *   ABL term : RK4 polynomial of the damped affine ODE where sigma
+ *              is constant over a step's reach (bulk and corners)  (pinned)
  *   ghosts   : brute-force periodic image / Dirichlet constants    (pinned)
  *   tensor C3 RHS (not in the reference): Fourier closed form      (pinned)
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -703,3 +706,7 @@ int or_adjoint_step(or_solver *s, double *lam) {
                     lam[(((int64_t)c * nz + k) * ny + j) * nx + i] = AT(s, ACC, i, j, k, c);
     return 0;
 }
+
+/* OpenMP threads used by the stepping loops (CPU-baseline timing) */
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_max_threads(void) { return omp_get_max_threads(); }

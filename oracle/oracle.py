@@ -15,6 +15,8 @@ pinned by the checks in tests/test_oracle.py:
   - RK4 / Euler / KO / every registered RHS on periodic grids against the
     single-Fourier-mode closed form G(k)^N (oracle/fourier.py);
   - ABL sigma and the ghost fill against brute force on tiny grids;
+  - the ABL damping term against the RK4 polynomial of the damped affine ODE
+    (constant fields where sigma is constant over a step's reach);
   - bitwise invariances (thread count).
 Parts the Fourier and brute-force checks cannot reach -- AMReX internals such
 as FillPatch ordering on multi-box levels -- stay "parity unpinned".
@@ -105,6 +107,8 @@ def lib() -> C.CDLL:
         h.or_ko_coef.argtypes = [C.c_int, C.c_int]
         h.or_adjoint_step.restype = C.c_int
         h.or_adjoint_step.argtypes = [VP, DP]
+        h.or_set_threads.argtypes = [C.c_int]
+        h.or_max_threads.restype = C.c_int
         h.or_system_ncomp.restype = C.c_int
         h.or_system_ncomp.argtypes = [C.c_int]
         _lib = h
@@ -244,4 +248,10 @@ def ko_coefs(r: int):
 
 
 def set_threads(n: int):
+    """OpenMP threads of the stepping loops from now on."""
     os.environ["OMP_NUM_THREADS"] = str(n)
+    lib().or_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return lib().or_max_threads()

@@ -145,6 +145,33 @@ def test_graph_replay_equals_eager(nsteps):
     assert np.array_equal(got, o.get_state())
 
 
+def test_graph_prepare_and_mixed_schemes():
+    """Graphs are keyed by the register roles they were captured with: prepared
+    graphs of both parities, odd remainders, and Euler steps (which rotate the
+    roles differently) between graph replays all equal the oracle."""
+    _say("[parity] graph prepare, Euler between RK4 graph replays")
+    case, state = small_case("phipi", n=(40, 24, 20), stencil_n=2, ko_order=3, periodic=(1, 0, 1), abl=3)
+    s = Solver(case)
+    s.set_state(state)
+    s.graph_prepare()
+    s.graph_rk4_steps(3)
+    s.euler_steps(1)
+    s.graph_rk4_steps(4)
+    s.graph_prepare()
+    s.euler_steps(1)
+    s.graph_rk4_steps(5)
+    got = s.get_state()
+    o = orc.OracleSolver(case)
+    o.set_state(state)
+    o.rk4_steps(3)
+    o.euler_steps(1)
+    o.rk4_steps(4)
+    o.euler_steps(1)
+    o.rk4_steps(5)
+    assert np.array_equal(got, o.get_state())
+    assert s.steps == o.steps == 14 and s.time == o.time
+
+
 @pytest.mark.parametrize("per,bc", [((1, 1, 1), "noop"), ((0, 0, 0), "dirichlet"), ((1, 0, 1), "dirichlet")])
 def test_ghost_fill_matches_oracle(per, bc):
     _say(f"[parity] ghost fill periodic={per} bc={bc}")

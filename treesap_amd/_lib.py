@@ -109,6 +109,7 @@ PROTOTYPES = {
     "tsp_set_kernel": (C.c_int, [_VP, C.POINTER(TspKernelCfg)]),
     "tsp_autotune": (C.c_int, [_VP, C.c_int32, C.POINTER(TspKernelCfg)]),
     "tsp_graph_rk4_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_graph_prepare": (C.c_int, [_VP]),
     "tsp_profile_rk4": (C.c_int, [_VP, C.c_int64, _DP, _DP]),
     "tsp_time_rk4": (C.c_int, [_VP, C.c_int64, C.c_int32, _DP]),
     "tsp_sync": (C.c_int, [_VP]),

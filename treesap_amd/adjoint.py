@@ -20,6 +20,14 @@ from . import _lib
 from .solver import Solver, _is_torch_cuda, _torch_ready
 
 
+def _check_cuda(t, shape, name):
+    """A device tensor handed to the library: contiguous float64 of exactly `shape` (the library
+    reads or writes that many doubles through its pointer)."""
+    if tuple(t.shape) != tuple(shape) or str(t.dtype) != "torch.float64" or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float64 tensor of shape {tuple(shape)}, got "
+                         f"{tuple(t.shape)} {t.dtype}{'' if t.is_contiguous() else ' (non-contiguous)'}")
+
+
 class Adjoint:
     def __init__(self, solver: Solver, receivers, nsteps: int, checkpoint_every: int = 50):
         rec = np.ascontiguousarray(np.asarray(receivers, dtype=np.int32).reshape(-1, 3))
@@ -55,6 +63,7 @@ class Adjoint:
 
     def trace(self, out=None):
         if out is not None and _is_torch_cuda(out):
+            _check_cuda(out, (self.nsteps, self.nrec), "out")
             _torch_ready(out)
             _lib.check(_lib.lib().tsp_adjoint_trace(self._h, C.c_void_p(out.data_ptr()), 1), "tsp_adjoint_trace")
             return out
@@ -72,6 +81,7 @@ class Adjoint:
         """dJ/du^0 for dtrace[n-1, r] = dJ/dphi^n(receiver r)."""
         if _is_torch_cuda(dtrace):
             dtrace = dtrace.contiguous()
+            _check_cuda(dtrace, (self.nsteps, self.nrec), "dtrace")
             _torch_ready(dtrace)
             dptr, dev = C.c_void_p(dtrace.data_ptr()), 1
         else:
@@ -80,6 +90,7 @@ class Adjoint:
                 raise ValueError(f"dtrace must have shape {(self.nsteps, self.nrec)}")
             dptr, dev = dtrace.ctypes.data_as(C.c_void_p), 0
         if out is not None and _is_torch_cuda(out):
+            _check_cuda(out, self.solver.case.shape, "out")
             _torch_ready(out)
             _lib.check(_lib.lib().tsp_adjoint_backward(self._h, dptr, dev, C.c_void_p(out.data_ptr()), 1,
                                                        int(bool(recompute))), "tsp_adjoint_backward")

@@ -51,8 +51,11 @@ struct tsp_solver {
     double time = 0.0;
     int64_t steps = 0;
     double gf1[3]{}, gf2[3]{};
-    // hipGraph of two RK4 steps per starting parity, valid for `graph_cfg`
+    // hipGraphs of two RK4 steps, each for the register roles it was captured
+    // with (two steps bring the roles back), valid for `graph_cfg`
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_role[2][4] = {};
+    int graph_next = 0; // slot the next capture replaces when both are taken
     tsp_kernel_cfg graph_cfg{};
 };
 
@@ -106,6 +109,9 @@ bool pairs_apply(const tsp_solver *s, int ghost_mode, std::string *why) {
         return no("stage_pairs needs registers under 4 GiB");
     for (int a = 0; a < 3; ++a)
         if (s->d.periodic[a] && s->d.n[a] < 2 * s->H) return no("stage_pairs needs periodic axes of at least 2H cells");
+    if (tsp::pair_lds_bytes_phipi(s->d.stencil_n, s->d.ko_order) + tsp::sigma_lds_bytes(s->d.abl_thickness) >
+        tsp::max_block_lds())
+        return no("stage_pairs: the pair kernel's LDS and the ABL rate table exceed the device's LDS per workgroup");
     return true;
 }
 
@@ -208,8 +214,10 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
         p2.b1 = dt * kBFrac[3];
         p2.a_next1 = 0.0;
         if (tm) TSP_HIP(hipEventRecord(tm->stage[0].start, s->stream));
+        // tsp_set_kernel admitted the mode (pairs_apply), so the pair kernel must run
         hipError_t e = s->launch_pair(p0, 0, s->stream);
-        if (e != hipErrorNotSupported) { // otherwise nothing was launched: run the single stages
+        if (e == hipErrorNotSupported) return fail(TSP_UNSUPPORTED, "stage-pair kernel does not apply to this solver");
+        {
             TSP_HIP(e);
             if (tm) {
                 TSP_HIP(hipEventRecord(tm->stage[0].stop, s->stream));
@@ -261,6 +269,53 @@ tsp_status enqueue_euler_step(tsp_solver *s) {
 void advance_clock(tsp_solver *s) { // Integrator.hpp:98-100
     s->steps++;
     s->time += s->d.dt;
+}
+
+// The captured two-step graph for register roles `roles` (capturing it if
+// missing); graphs of another kernel configuration are dropped first.
+tsp_status ensure_graph(tsp_solver *s, const int roles[4], int *slot) {
+    if (std::memcmp(&s->graph_cfg, &s->cfg, sizeof(tsp_kernel_cfg)) != 0) {
+        drop_graphs(s);
+        s->graph_cfg = s->cfg;
+    }
+    for (int g = 0; g < 2; ++g)
+        if (s->graph[g] && std::memcmp(s->graph_role[g], roles, sizeof(s->graph_role[g])) == 0) {
+            *slot = g;
+            return TSP_OK;
+        }
+    const int g = !s->graph[0] ? 0 : !s->graph[1] ? 1 : s->graph_next;
+    s->graph_next = 1 - g;
+    if (s->graph[g]) {
+        (void)hipGraphExecDestroy(s->graph[g]);
+        s->graph[g] = nullptr;
+    }
+    int saved[4];
+    std::memcpy(saved, s->role, sizeof(saved));
+    std::memcpy(s->role, roles, sizeof(saved));
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        std::memcpy(s->role, saved, sizeof(saved));
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    tsp_status st = enqueue_rk4_step(s, nullptr);
+    if (st == TSP_OK) st = enqueue_rk4_step(s, nullptr);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s->stream, &graph);
+    std::memcpy(s->role, saved, sizeof(saved));
+    if (st != TSP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&s->graph[g], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        s->graph[g] = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    std::memcpy(s->graph_role[g], roles, sizeof(s->graph_role[g]));
+    *slot = g;
+    return TSP_OK;
 }
 
 } // namespace
@@ -516,40 +571,30 @@ tsp_status tsp_register_view(tsp_solver *s, int32_t role, double **ptr, int64_t 
 tsp_status tsp_graph_rk4_steps(tsp_solver *s, int64_t nsteps) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
-    if (std::memcmp(&s->graph_cfg, &s->cfg, sizeof(tsp_kernel_cfg)) != 0) {
-        drop_graphs(s);
-        s->graph_cfg = s->cfg;
-    }
-    const int parity = (int)(s->steps & 1);
-    if (!s->graph[parity] && nsteps >= 2) {
-        int saved[4];
-        std::memcpy(saved, s->role, sizeof(saved));
-        TSP_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        tsp_status st = enqueue_rk4_step(s, nullptr);
-        if (st == TSP_OK) st = enqueue_rk4_step(s, nullptr);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(s->stream, &graph);
-        std::memcpy(s->role, saved, sizeof(saved)); // two steps bring the roles back
-        if (st != TSP_OK) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return st;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&s->graph[parity], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            s->graph[parity] = nullptr;
-            return hip_fail(e, "hipGraphInstantiate");
-        }
+    int slot = -1;
+    if (nsteps >= 2) {
+        tsp_status st = ensure_graph(s, s->role, &slot);
+        if (st != TSP_OK) return st;
     }
     int64_t n = 0;
     for (; n + 2 <= nsteps; n += 2) {
-        TSP_HIP(hipGraphLaunch(s->graph[parity], s->stream));
+        TSP_HIP(hipGraphLaunch(s->graph[slot], s->stream));
         advance_clock(s);
         advance_clock(s);
     }
     if (n < nsteps) return tsp_rk4_steps(s, nsteps - n);
     return TSP_OK;
+}
+
+tsp_status tsp_graph_prepare(tsp_solver *s) {
+    if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
+    int slot = -1;
+    tsp_status st = ensure_graph(s, s->role, &slot);
+    if (st != TSP_OK) return st;
+    // roles one RK4 step later: old <-> new, arg <-> arg_next
+    const int next[4] = {s->role[TSP_ROLE_NEW], s->role[TSP_ROLE_OLD], s->role[TSP_ROLE_ARG_NEXT],
+                         s->role[TSP_ROLE_ARG]};
+    return ensure_graph(s, next, &slot);
 }
 
 tsp_status tsp_time_rk4(tsp_solver *s, int64_t nsteps, int32_t use_graph, double *ms) {

@@ -56,10 +56,11 @@ inline tsp::LaunchFn lookup_launcher(int system, int n, int r) {
     }
 }
 
-// Launcher specialised for forward RK4 stage `stage` (0 or 3), or null.
+// Launcher specialised for forward RK4 stage `stage`, or null.
 inline tsp::LaunchFn lookup_stage_launcher(int system, int n, int r, int stage) {
     switch (system) {
-    case TSP_SYS_WAVE_PHIPI: return stage == 0 ? tsp::lookup_phipi_s0(n, r) : stage == 3 ? tsp::lookup_phipi_s3(n, r) : nullptr;
+    case TSP_SYS_WAVE_PHIPI:
+        return stage == 0 ? tsp::lookup_phipi_s0(n, r) : stage == 3 ? tsp::lookup_phipi_s3(n, r) : tsp::lookup_phipi_s1(n, r);
     case TSP_SYS_WAVE_PHIU: return stage == 0 ? tsp::lookup_phiu_s0(n, r) : stage == 3 ? tsp::lookup_phiu_s3(n, r) : nullptr;
     case TSP_SYS_NONLINEAR:
         return stage == 0 ? tsp::lookup_nonlinear_s0(n, r) : stage == 3 ? tsp::lookup_nonlinear_s3(n, r) : nullptr;

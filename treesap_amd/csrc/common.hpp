@@ -146,9 +146,13 @@ LaunchFn lookup_tensor(int n, int r);
 // first = 0 or 2); hipErrorNotSupported, without launching, where it does not apply.
 using PairFn = hipError_t (*)(const StageArgs &a, int first, hipStream_t stream);
 PairFn lookup_pair_phipi(int n, int r);
-// launchers specialised for forward RK4 stage 0 / stage 3 (hipErrorNotSupported
-// for configurations they do not cover: the caller then uses the generic one)
+// static LDS bytes of the (phi, Pi) stage-pair kernel (0 where none is built)
+size_t pair_lds_bytes_phipi(int n, int r);
+// launchers specialised for forward RK4 stage 0 / stage 3 / the middle stages
+// 1 and 2 (hipErrorNotSupported for configurations they do not cover: the
+// caller then uses the generic one)
 LaunchFn lookup_phipi_s0(int n, int r);
+LaunchFn lookup_phipi_s1(int n, int r);
 LaunchFn lookup_phipi_s3(int n, int r);
 LaunchFn lookup_phiu_s0(int n, int r);
 LaunchFn lookup_phiu_s3(int n, int r);
@@ -180,5 +184,24 @@ hipError_t launch_inject_trace(double *lam, const int *rec_off, int nrec, const 
                                int *counter, hipStream_t stream);
 hipError_t launch_set_counter(int *counter, int v, hipStream_t stream);
 hipError_t launch_zero(double *p, long long n, hipStream_t stream);
+
+// dynamic LDS bytes for the ABL rate table of thickness T
+inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
+
+// LDS a block may hold (static + dynamic), queried once per device
+inline size_t max_block_lds() {
+    static thread_local int dev_cached = -1;
+    static thread_local size_t bytes = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (dev != dev_cached) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 0;
+        bytes = (size_t)v;
+        dev_cached = dev;
+    }
+    return bytes;
+}
+
 
 } // namespace tsp

@@ -18,4 +18,18 @@ PairFn lookup_pair_phipi(int n, int r) {
     return nullptr;
 }
 
+size_t pair_lds_bytes_phipi(int n, int r) {
+#define TSP_CASE(NN, RR)                                                                                              \
+    if (n == NN && r == RR) return sizeof(double) * StagePair<SysPhiPi, NN, RR, 0>::LDS_DOUBLES;
+    TSP_CASE(1, 1)
+    TSP_CASE(1, 2)
+    TSP_CASE(2, 2)
+    TSP_CASE(2, 3)
+    TSP_CASE(3, 3)
+    TSP_CASE(3, 4)
+    TSP_CASE(4, 4)
+#undef TSP_CASE
+    return 0;
+}
+
 } // namespace tsp

@@ -288,6 +288,10 @@ struct FusedStage {
 
     using Plane = double[LH][LW];
 
+    // Lookahead of the stage-specific forward bodies (run_tile2: two steps for
+    // queue heads and halos, one for centre values); run_tile1 elsewhere.
+    static constexpr int LA = (SM >= 0 && !ADJ) ? 2 : 1;
+
     // DOWN: the tile marches towards decreasing z; the queue then holds
     // q[m] = plane z + H - m instead of z - H + m.
     template <bool DOWN> struct View {
@@ -345,6 +349,13 @@ struct FusedStage {
     template <bool DOWN>
     __device__ __forceinline__ static void run_tile(const StageArgs &a, int x0, int y0, int z0, int z1,
                                                     double *lds_raw, const double *sig) {
+        if constexpr (LA == 2) run_tile2<DOWN>(a, x0, y0, z0, z1, lds_raw, sig);
+        else run_tile1<DOWN>(a, x0, y0, z0, z1, lds_raw, sig);
+    }
+
+    template <bool DOWN>
+    __device__ __forceinline__ static void run_tile1(const StageArgs &a, int x0, int y0, int z0, int z1,
+                                                     double *lds_raw, const double *sig) {
         const int tid = threadIdx.x;
         const int lane = tid & (TX - 1);
         const int row = tid / TX;
@@ -547,6 +558,197 @@ struct FusedStage {
         }
     }
 
+    // run_tile with a two-step lookahead, for the stage-specific forward
+    // bodies: the queue head and x/y halo a step consumes were requested two
+    // steps earlier and its centre init/acc values one step earlier, in
+    // alternating register sets (the plane loop is unrolled by two so that
+    // every set has a fixed register name), so no wait in step t targets a
+    // load issued in step t and twice as many bytes are in flight.
+    template <bool DOWN>
+    __device__ __forceinline__ static void run_tile2(const StageArgs &a, int x0, int y0, int z0, int z1,
+                                                     double *lds_raw, const double *sig) {
+        static_assert(SM >= 0 && !ADJ, "two-step lookahead is built for the forward stage-specific bodies");
+        const int tid = threadIdx.x;
+        const int lane = tid & (TX - 1);
+        const int row = tid / TX;
+        const int gi = x0 + lane, gj = y0 + row;
+        const bool active = gi < a.nx && gj < a.ny;
+        const int cnt = z1 - z0;
+        const int tlast = cnt - 1;
+
+        constexpr bool is_first = SM == kStage0 || SM == kEuler;
+        constexpr bool write_argn = SM < kStage3;
+        constexpr bool read_init = write_argn && !is_first;
+        constexpr bool read_acc = !is_first;
+        const int wrapx = a.ghost_mode == 0 && a.periodic[0];
+        const int wrapy = a.ghost_mode == 0 && a.periodic[1];
+        const int wrapz = a.ghost_mode == 0 && a.periodic[2];
+        const Access mem(a);
+        auto zoff_of = [&](int k) { return mem.plane(map_axis(k, a.nz, wrapz, a.G)); };
+        auto plane = [&](int t) { return DOWN ? z1 - 1 - t : z0 + t; };
+        auto qplane = [&](int t) { return DOWN ? z1 - 1 - t - H : z0 + t + H; };
+
+        const int qcol = map_axis(gi, a.nx, wrapx, a.G) + map_axis(gj, a.ny, wrapy, a.G) * a.sj;
+        const int ccol = active ? gi + gj * a.sj : qcol;
+        const auto qh = mem.col(qcol), ch = mem.col(ccol);
+        decltype(mem.col(0)) hh[HPT];
+        int hlds[HPT];
+        bool hok[HPT];
+#pragma unroll
+        for (int m = 0; m < HPT; ++m) {
+            const int h = tid + m * NT;
+            int hx, hy;
+            if (h < 2 * H * TX) {
+                const int r = h / TX;
+                hy = r < H ? r - H : TY + (r - H);
+                hx = h % TX;
+            } else {
+                const int e = h - 2 * H * TX;
+                const int mm = e % (2 * H);
+                hy = e / (2 * H);
+                hx = mm < H ? mm - H : TX + (mm - H);
+            }
+            hok[m] = h < NHALO;
+            const int hcol =
+                hok[m] ? map_axis(x0 + hx, a.nx, wrapx, a.G) + map_axis(y0 + hy, a.ny, wrapy, a.G) * a.sj : qcol;
+            hh[m] = mem.col(hcol);
+            hlds[m] = (hy + H) * LW + hx + H;
+        }
+
+        int cxy2 = 0;
+        const int T = a.abl_T;
+        if (T > 0) {
+            const int dxv = gi - min(max(gi, T), a.nx - 1 - T);
+            const int dyv = gj - min(max(gj, T), a.ny - 1 - T);
+            cxy2 = dxv * dxv + dyv * dyv;
+        }
+
+        double q[NS][Q], qn[2][NS], hv[2][NS][HPT], iv[2][NC], av[2][NC], cv[2][NC];
+
+        auto load_halo = [&](int t, double (&dst)[NS][HPT]) {
+            const auto zh = zoff_of(plane(t));
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (Staging<Sys, R>::staged(C)) {
+                    constexpr int sl = Staging<Sys, R>::slot(C);
+#pragma unroll
+                    for (int m = 0; m < HPT; ++m) dst[sl][m] = src_at<C>(a, mem, hh[m], zh);
+                }
+            });
+        };
+        auto load_queue = [&](int t, double (&dst)[NS]) {
+            const auto zq = zoff_of(qplane(t));
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (Staging<Sys, R>::staged(C)) dst[Staging<Sys, R>::slot(C)] = src_at<C>(a, mem, qh, zq);
+            });
+        };
+        auto load_centre = [&](int t, double (&ivd)[NC], double (&avd)[NC], double (&cvd)[NC]) {
+            const auto zp = mem.plane(plane(t));
+            static_for<0, NC>([&]<int C>() {
+                if constexpr (!Staging<Sys, R>::staged(C)) cvd[C] = src_at<C>(a, mem, ch, zp);
+                if constexpr (read_init) ivd[C] = mem.template init<C>(ch, zp);
+                if constexpr (read_acc) avd[C] = mem.template acc<C>(ch, zp);
+            });
+        };
+        auto put_halo = [&](int slot, const double (&src)[NS][HPT]) {
+            double *dst = lds_raw + slot * LDS_PLANE;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int m = 0; m < HPT; ++m)
+                    if (hok[m]) dst[s * LH * LW + hlds[m]] = src[s][m];
+        };
+
+        // prologue: queue planes before step 0, heads of steps 0 and 1, centre
+        // values of step 0; halos of steps 0 and 1 in flight, or with the ring
+        // those of steps 0..RS-1 in their buffers and of steps RS, RS+1 in flight
+        static_for<0, NC>([&]<int C>() {
+            if constexpr (Staging<Sys, R>::staged(C)) {
+                constexpr int sl = Staging<Sys, R>::slot(C);
+                q[sl][0] = 0.0;
+#pragma unroll
+                for (int m = 1; m < Q; ++m)
+                    q[sl][m] = src_at<C>(a, mem, qh, zoff_of(DOWN ? z1 + H - m : z0 - H - 1 + m));
+            }
+        });
+        load_queue(0, qn[0]);
+        load_queue(min(1, tlast), qn[1]);
+        load_centre(0, iv[0], av[0], cv[0]);
+        if constexpr (RS > 0) {
+#pragma unroll
+            for (int p = 0; p < RS; ++p) {
+                load_halo(min(p, tlast), hv[0]);
+                put_halo(p, hv[0]);
+            }
+            load_halo(min(RS, tlast), hv[0]);
+            load_halo(min(RS + 1, tlast), hv[1]);
+        } else {
+            load_halo(0, hv[0]);
+            load_halo(min(1, tlast), hv[1]);
+        }
+        int cslot = 0;
+        int hslot = RS % NSLOT;
+
+        auto step = [&]<int P>(int t) {
+            const int z = plane(t);
+            const auto zp = mem.plane(z);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+#pragma unroll
+                for (int m = 0; m < Q - 1; ++m) q[s][m] = q[s][m + 1];
+                q[s][Q - 1] = qn[P][s];
+            }
+            load_centre(min(t + 1, tlast), iv[1 - P], av[1 - P], cv[1 - P]);
+            load_queue(min(t + 2, tlast), qn[P]);
+            double *buf = lds_raw + cslot * LDS_PLANE;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) buf[s * LH * LW + (row + H) * LW + lane + H] = q[s][H];
+            put_halo(hslot, hv[P]);
+            if constexpr (RS > 0) load_halo(min(t + RS + 2, tlast), hv[P]);
+            else load_halo(min(t + 2, tlast), hv[P]);
+            cslot = cslot == NSLOT - 1 ? 0 : cslot + 1;
+            hslot = hslot == NSLOT - 1 ? 0 : hslot + 1;
+            __syncthreads();
+
+            if (active) {
+                double cen[NC];
+                static_for<0, NC>([&]<int C>() {
+                    if constexpr (Staging<Sys, R>::staged(C)) cen[C] = q[Staging<Sys, R>::slot(C)][H];
+                    else cen[C] = cv[P][C];
+                });
+                double sigma = 0.0;
+                if (T > 0) {
+                    const int dzv = z - min(max(z, T), a.nz - 1 - T);
+                    const int d2 = cxy2 + dzv * dzv;
+                    sigma = d2 == 0 ? 0.0 : sig[d2];
+                }
+                const View<DOWN> v{q, reinterpret_cast<const Plane *>(buf), cen, a, lane + H, row + H};
+                static_for<0, NC>([&]<int C>() {
+                    double k = Sys::template user<N, C>(v, a);
+                    if constexpr (R > 0) {
+                        double ko_term = 0.0;
+                        ko_term += KO<R, 0, C>(v);
+                        ko_term += KO<R, 1, C>(v);
+                        ko_term += KO<R, 2, C>(v);
+                        k = k + a.ko_strength * ko_term;
+                    }
+                    if (T > 0) k = k - sigma * (cen[C] - a.target[C]);
+                    const double initc = read_init ? iv[P][C] : cen[C];
+                    double accv;
+                    if constexpr (is_first) accv = initc + a.b * k;
+                    else accv = av[P][C] + a.b * k;
+                    mem.template put_acc<C>(accv, ch, zp);
+                    if constexpr (write_argn) mem.template put_argn<C>(initc + a.a_next * k, ch, zp);
+                });
+            }
+        };
+        int t = 0;
+        for (; t + 1 < cnt; t += 2) {
+            step.template operator()<0>(t);
+            step.template operator()<1>(t + 1);
+        }
+        if (t < cnt) step.template operator()<0>(t);
+    }
+
     // Copy the ABL rate table (3 T^2 + 1 entries) into LDS; visible to the
     // block after the first barrier of run_tile.
     __device__ __forceinline__ static const double *stage_sigma(const double *tab, int T, double *dst) {
@@ -581,8 +783,6 @@ struct FusedStage {
     }
 };
 
-// dynamic LDS bytes for the ABL rate table of thickness T
-inline size_t sigma_lds_bytes(int T) { return T > 0 ? sizeof(double) * (size_t)(3 * T * T + 1) : 0; }
 
 template <class Sys, int N, int R, int TY, bool ADJ, int RS = 0, bool BUF = true, int TXW = 64, int SM = -1>
 __global__ __launch_bounds__(TXW * TY) void fused_stage_kernel(const StageArgs a) {
@@ -593,21 +793,6 @@ __global__ __launch_bounds__(TXW * TY) void fused_stage_kernel(const StageArgs a
 // (one block per CU; five plane buffers), 2 for 64x8 tiles (four buffers, so
 // that two blocks share a CU's LDS).
 template <int TY, int H> constexpr int ring_slots() { return TY >= 16 ? 3 : TY == 8 ? 2 : 0; }
-
-// LDS a block may hold (static + dynamic), queried once per device
-inline size_t max_block_lds() {
-    static thread_local int dev_cached = -1;
-    static thread_local size_t bytes = 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (dev != dev_cached) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 0;
-        bytes = (size_t)v;
-        dev_cached = dev;
-    }
-    return bytes;
-}
 
 // Tile heights with a halo-ring variant: use it when asked for and when its
 // LDS (static + the dynamic ABL table) fits a block on this device.

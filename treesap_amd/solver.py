@@ -78,8 +78,8 @@ class Solver:
     def get_state(self, out=None):
         """state_old (valid region) as a numpy array, or into a float64 torch CUDA tensor."""
         if out is not None and _is_torch_cuda(out):
-            if not out.is_contiguous() or tuple(out.shape) != self.case.shape:
-                raise ValueError("out must be contiguous with the state shape")
+            if not out.is_contiguous() or tuple(out.shape) != self.case.shape or str(out.dtype) != "torch.float64":
+                raise ValueError("out must be a contiguous float64 tensor of the state shape")
             _torch_ready(out)
             _lib.check(_lib.lib().tsp_get_state(self._h, C.c_void_p(out.data_ptr()), 1), "tsp_get_state")
             return out
@@ -107,6 +107,10 @@ class Solver:
 
     def graph_rk4_steps(self, nsteps: int) -> None:
         _lib.check(_lib.lib().tsp_graph_rk4_steps(self._h, int(nsteps)), "tsp_graph_rk4_steps")
+
+    def graph_prepare(self) -> None:
+        """Capture the two-step graphs of both step parities (nothing is launched)."""
+        _lib.check(_lib.lib().tsp_graph_prepare(self._h), "tsp_graph_prepare")
 
     def step(self) -> None:
         """Integrator::step() with the RK4 scheme."""
@@ -214,7 +218,15 @@ class Batch:
         _lib.check(_lib.lib().tsp_batch_set_state(self._h, index, arr.ctypes.data_as(C.c_void_p), 0),
                    "tsp_batch_set_state")
 
-    def get_state(self, index: int) -> np.ndarray:
+    def get_state(self, index: int, out=None):
+        """State of instance `index` as a numpy array, or into a float64 torch CUDA tensor."""
+        if out is not None and _is_torch_cuda(out):
+            if not out.is_contiguous() or tuple(out.shape) != self.cases[index].shape or str(out.dtype) != "torch.float64":
+                raise ValueError("out must be a contiguous float64 tensor of the instance's state shape")
+            _torch_ready(out)
+            _lib.check(_lib.lib().tsp_batch_get_state(self._h, index, C.c_void_p(out.data_ptr()), 1),
+                       "tsp_batch_get_state")
+            return out
         arr = np.empty(self.cases[index].shape, dtype=np.float64)
         _lib.check(_lib.lib().tsp_batch_get_state(self._h, index, arr.ctypes.data_as(C.c_void_p), 0),
                    "tsp_batch_get_state")
