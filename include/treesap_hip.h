@@ -229,6 +229,9 @@ tsp_status tsp_batch_set_kernel(tsp_batch *b, const tsp_batch_cfg *cfg);
  * steps per graph, odd remainder eager). */
 tsp_status tsp_batch_rk4_steps(tsp_batch *b, int64_t nsteps);
 tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
+/* Capture and instantiate, without launching, the two-step graphs of both step
+ * parities for the current configuration. */
+tsp_status tsp_batch_graph_prepare(tsp_batch *b);
 tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms);
 tsp_status tsp_batch_sync(tsp_batch *b);
 
@@ -268,6 +271,24 @@ tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dt
 /* One transposed RK4 step, lam_out = G^T lam_in (valid layout), for
  * dot-product checks. */
 tsp_status tsp_adjoint_apply(tsp_adjoint *a, const double *lam_in, double *lam_out, int32_t on_device);
+
+/* Discrete adjoint of a whole batch (no reference counterpart): the single-box
+ * adjoint above for every instance at once, each launch covering all
+ * instances (one receiver gather or inject, one reverse-stage launch per
+ * variant group). Bound to one batch (borrowed; it must outlive it); the nrec
+ * receivers (i, j, k) are shared by all instances and must lie inside every
+ * instance's box. Traces are [instance][step][receiver]; gradients and dtrace
+ * follow the same instance-major order, gradients in the valid layout of
+ * tsp_batch_get_state concatenated over instances. Every instance's trace and
+ * gradient are bit-identical to a single-box tsp_adjoint on that instance. */
+typedef struct tsp_batch_adjoint tsp_batch_adjoint;
+tsp_status tsp_batch_adjoint_create(tsp_batch *b, const int32_t *receivers, int32_t nrec, int32_t nsteps,
+                                    int32_t checkpoint_every, tsp_batch_adjoint **out);
+tsp_status tsp_batch_adjoint_destroy(tsp_batch_adjoint *a);
+tsp_status tsp_batch_adjoint_forward(tsp_batch_adjoint *a);
+tsp_status tsp_batch_adjoint_trace(tsp_batch_adjoint *a, double *dst, int32_t on_device);
+tsp_status tsp_batch_adjoint_backward(tsp_batch_adjoint *a, const double *dtrace, int32_t dtrace_on_device,
+                                      double *grad, int32_t grad_on_device, int32_t recompute);
 
 /* Algorithmic HBM bytes of one RK4 step per valid cell (128 * ncomp). */
 int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d);

@@ -216,3 +216,63 @@ def test_backward_with_stage_pairs_matches_oracle(ko, per, abl):
     dtrace = 2.0 * (trace - target)
     assert np.array_equal(a.backward(dtrace), ograd)
     assert np.array_equal(a.backward(dtrace, recompute=True), ograd)
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_batch_adjoint_equals_single_box_adjoints(recompute):
+    """BatchAdjoint over four sources of one model (odd step count, checkpoint interval not
+    dividing it, receivers sharing a cell): every instance's trace and gradient equal the
+    single-box adjoint of that instance bitwise, stored and recomputed."""
+    from treesap_amd import Batch
+    from treesap_amd.adjoint import BatchAdjoint
+
+    _say(f"[adjoint] batch adjoint recompute={recompute}")
+    case, st0, rec, target, nsteps, K = _problem(n=(36, 30, 28), sn=2, ko=2, abl=4, nsteps=11, K=4)
+    states = []
+    for k in range(4):
+        _, st = small_case("phipi", n=case.n, stencil_n=2, ko_order=2, periodic=case.periodic, abl=4, seed=40 + k)
+        st[1] = 0.0
+        states.append(st)
+    b = Batch([case] * 4)
+    for i, st in enumerate(states):
+        b.set_state(i, st)
+    ba = BatchAdjoint(b, rec, nsteps, K)
+    ba.forward()
+    tr = ba.trace()
+    dtr = 2.0 * (tr - target[None])
+    g = ba.backward(dtr, recompute=recompute).reshape(4, *case.shape)
+    for i, st in enumerate(states):
+        s = Solver(case)
+        s.set_state(st)
+        a = Adjoint(s, rec, nsteps, K)
+        a.forward()
+        assert np.array_equal(a.trace(), tr[i]), i
+        gi = a.backward(dtr[i], recompute=recompute)
+        assert np.array_equal(gi, g[i]), i
+        a.close()
+        s.close()
+    # the batch is left at the end of the forward pass
+    assert b.steps == nsteps
+    ba.close()
+    b.close()
+
+
+def test_adjoint_rejects_bad_tensors():
+    """Device tensors handed to the adjoint must be contiguous float64 of the exact shape."""
+    torch = pytest.importorskip("torch")
+    case, st, rec, target, nsteps, K = _problem()
+    s = Solver(case)
+    s.set_state(st)
+    a = Adjoint(s, rec, nsteps, K)
+    a.forward()
+    good = torch.zeros((nsteps, len(rec)), dtype=torch.float64, device="cuda")
+    for bad in (torch.zeros((nsteps, len(rec)), dtype=torch.float32, device="cuda"),
+                torch.zeros((nsteps + 1, len(rec)), dtype=torch.float64, device="cuda"),
+                torch.zeros((len(rec), nsteps), dtype=torch.float64, device="cuda").t()):
+        with pytest.raises(ValueError):
+            a.trace(out=bad)
+    with pytest.raises(ValueError):
+        a.backward(good, out=torch.zeros(case.shape, dtype=torch.float32, device="cuda"))
+    a.trace(out=good)
+    a.close()
+    s.close()

@@ -125,6 +125,7 @@ PROTOTYPES = {
     "tsp_batch_set_kernel": (C.c_int, [_VP, C.POINTER(TspBatchCfg)]),
     "tsp_batch_rk4_steps": (C.c_int, [_VP, C.c_int64]),
     "tsp_batch_graph_rk4_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_batch_graph_prepare": (C.c_int, [_VP]),
     "tsp_batch_time_rk4": (C.c_int, [_VP, C.c_int64, C.c_int32, _DP]),
     "tsp_batch_sync": (C.c_int, [_VP]),
     "tsp_adjoint_create": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, C.POINTER(_VP)]),
@@ -134,6 +135,12 @@ PROTOTYPES = {
     "tsp_adjoint_checkpoint": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32]),
     "tsp_adjoint_backward": (C.c_int, [_VP, _VP, C.c_int32, _VP, C.c_int32, C.c_int32]),
     "tsp_adjoint_apply": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
+    "tsp_batch_adjoint_create": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
+                                           C.POINTER(_VP)]),
+    "tsp_batch_adjoint_destroy": (C.c_int, [_VP]),
+    "tsp_batch_adjoint_forward": (C.c_int, [_VP]),
+    "tsp_batch_adjoint_trace": (C.c_int, [_VP, _VP, C.c_int32]),
+    "tsp_batch_adjoint_backward": (C.c_int, [_VP, _VP, C.c_int32, _VP, C.c_int32, C.c_int32]),
 }
 
 _lib = None

@@ -53,6 +53,7 @@ struct tsp_batch {
     int64_t steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1, -1};
+    uint64_t work_gen = 0; // bumped whenever the work list is rebuilt (graphs of other objects hold it)
 };
 
 namespace {
@@ -130,6 +131,7 @@ tsp_status build_work(tsp_batch *b) {
     TSP_HIP(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(tsp::BatchInstance) * b->count, hipMemcpyHostToDevice));
     b->nwork = (int)work.size();
     b->work_cfg = b->cfg;
+    b->work_gen++;
     return TSP_OK;
 }
 
@@ -164,6 +166,47 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
 void batch_advance_clock(tsp_batch *b) {
     b->steps++;
     for (int i = 0; i < b->count; ++i) b->time[i] += b->d[i].dt;
+}
+
+// register roles after an even / odd number of RK4 steps (one step swaps
+// old <-> new and arg <-> arg_next)
+constexpr int kEvenRoles[4] = {0, 1, 2, 3};
+constexpr int kOddRoles[4] = {1, 0, 3, 2};
+
+// The captured two-step graph for step parity `parity` (captured with that
+// parity's register roles if missing); graphs of another configuration are
+// dropped first.
+tsp_status ensure_batch_graph(tsp_batch *b, int parity) {
+    if (std::memcmp(&b->graph_cfg, &b->cfg, sizeof(tsp_batch_cfg)) != 0) {
+        drop_batch_graphs(b);
+        b->graph_cfg = b->cfg;
+    }
+    if (b->graph[parity]) return TSP_OK;
+    int saved[4];
+    std::memcpy(saved, b->role, sizeof(saved));
+    std::memcpy(b->role, parity ? kOddRoles : kEvenRoles, sizeof(saved));
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        std::memcpy(b->role, saved, sizeof(saved));
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    tsp_status st = enqueue_batch_step(b);
+    if (st == TSP_OK) st = enqueue_batch_step(b);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(b->stream, &graph);
+    std::memcpy(b->role, saved, sizeof(saved));
+    if (st != TSP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->graph[parity], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        b->graph[parity] = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return TSP_OK;
 }
 
 } // namespace
@@ -401,32 +444,8 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps) {
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
     tsp_status st = build_work(b);
     if (st != TSP_OK) return st;
-    if (std::memcmp(&b->graph_cfg, &b->cfg, sizeof(tsp_batch_cfg)) != 0) {
-        drop_batch_graphs(b);
-        b->graph_cfg = b->cfg;
-    }
     const int parity = (int)(b->steps & 1);
-    if (!b->graph[parity] && nsteps >= 2) {
-        int saved[4];
-        std::memcpy(saved, b->role, sizeof(saved));
-        TSP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        st = enqueue_batch_step(b);
-        if (st == TSP_OK) st = enqueue_batch_step(b);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &graph);
-        std::memcpy(b->role, saved, sizeof(saved));
-        if (st != TSP_OK) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return st;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&b->graph[parity], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            b->graph[parity] = nullptr;
-            return hip_fail(e, "hipGraphInstantiate");
-        }
-    }
+    if (nsteps >= 2 && (st = ensure_batch_graph(b, parity)) != TSP_OK) return st;
     int64_t n = 0;
     for (; n + 2 <= nsteps; n += 2) {
         TSP_HIP(hipGraphLaunch(b->graph[parity], b->stream));
@@ -435,6 +454,13 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps) {
     }
     if (n < nsteps) return tsp_batch_rk4_steps(b, nsteps - n);
     return TSP_OK;
+}
+
+tsp_status tsp_batch_graph_prepare(tsp_batch *b) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    tsp_status st = build_work(b);
+    for (int p = 0; p < 2 && st == TSP_OK; ++p) st = ensure_batch_graph(b, p);
+    return st;
 }
 
 tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms) {
@@ -462,6 +488,379 @@ tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, d
 tsp_status tsp_batch_sync(tsp_batch *b) {
     if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
     TSP_HIP(hipStreamSynchronize(b->stream));
+    return TSP_OK;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// Discrete adjoint of a batch: every instance's receiver loss gradient at once
+// ===========================================================================
+//
+// The same recursion as the single-box adjoint (capi.hip), with every launch
+// covering all instances: forward steps are the batch's RK4 steps followed by
+// one gather of every instance's receivers, reverse steps one inject and four
+// reverse-stage launches per variant group on the adjoint registers. Each
+// instance's trace and gradient are bit-identical to its single-box adjoint.
+
+struct tsp_batch_adjoint {
+    tsp_batch *b = nullptr;
+    int nrec = 0, nsteps = 0, K = 0, nck = 0;
+    int64_t step0 = 0;
+    std::vector<double> time0;
+    std::vector<int64_t> state_off; // element offset of instance i in a dense [instance][state] array
+    int64_t state_elems = 0;        // all instances
+    double *apool = nullptr;        // adjoint registers, the forward pool's layout
+    std::vector<tsp::BatchInstance> h_ainst;
+    tsp::BatchInstance *d_ainst = nullptr;
+    int *d_rec = nullptr;           // [instance][receiver] offsets of component 0 from the register origin
+    double *d_trace = nullptr;      // [instance][step][receiver]
+    double *d_dtrace = nullptr;
+    int *d_counter = nullptr;       // one trace row counter per instance
+    double *d_ckpt = nullptr;       // [slot][instance state]
+    int arole[4] = {0, 1, 2, 3};
+    hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
+    hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
+    tsp_batch_cfg gcfg{-1, -1, -1, -1, -1, -1};
+    uint64_t gwork = ~0ull;
+};
+
+namespace {
+
+void drop_batch_adjoint_graphs(tsp_batch_adjoint *a) {
+    for (auto *g : {&a->fgraph[0], &a->fgraph[1], &a->bgraph[0], &a->bgraph[1]})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
+// graphs hold the batch's work list and configuration
+tsp_status batch_adjoint_check(tsp_batch_adjoint *a) {
+    tsp_status st = build_work(a->b);
+    if (st != TSP_OK) return st;
+    if (a->gwork != a->b->work_gen || std::memcmp(&a->gcfg, &a->b->cfg, sizeof(tsp_batch_cfg)) != 0) {
+        drop_batch_adjoint_graphs(a);
+        a->gwork = a->b->work_gen;
+        a->gcfg = a->b->cfg;
+    }
+    return TSP_OK;
+}
+
+tsp_status enqueue_batch_fwd_gather(tsp_batch_adjoint *a) {
+    tsp_batch *b = a->b;
+    tsp_status st = enqueue_batch_step(b);
+    if (st != TSP_OK) return st;
+    TSP_HIP(tsp::launch_batch_gather_trace(b->d_inst, b->count, b->role[TSP_ROLE_OLD], a->d_rec, a->nrec, a->d_trace,
+                                           a->nsteps, a->d_counter, b->stream));
+    return TSP_OK;
+}
+
+// lambda <- G^T lambda for every instance (lambda in adjoint role OLD)
+tsp_status enqueue_batch_adjoint_step(tsp_batch_adjoint *a) {
+    tsp_batch *b = a->b;
+    int &OLD = a->arole[TSP_ROLE_OLD], &NEW = a->arole[TSP_ROLE_NEW], &ARG = a->arole[TSP_ROLE_ARG],
+        &ARGN = a->arole[TSP_ROLE_ARG_NEXT];
+    for (int j = 3; j >= 0; --j) {
+        if (j < 3) std::swap(ARG, ARGN);
+        for (int v = 0; v < tsp::kBatchVariants; ++v) {
+            if (b->group_n[v] == 0) continue;
+            tsp::BatchArgs g{};
+            g.inst = a->d_ainst;
+            g.work = b->d_work + b->group_off[v];
+            g.nwork = b->group_n[v];
+            g.stage = j;
+            g.src = j == 3 ? OLD : ARG;
+            g.init = OLD;
+            g.acc = NEW;
+            g.argn = j > 0 ? ARGN : -1;
+            g.zalt = b->cfg.zalt;
+            g.ghost_mode = 0;
+            const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, g.nwork) : g.nwork;
+            TSP_HIP(tsp::launch_batched_phipi_adjoint(v, g, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring,
+                                                      b->stream));
+        }
+    }
+    std::swap(OLD, NEW);
+    return TSP_OK;
+}
+
+tsp_status enqueue_batch_inject_adjoint(tsp_batch_adjoint *a) {
+    TSP_HIP(tsp::launch_batch_inject_trace(a->d_ainst, a->b->count, a->arole[TSP_ROLE_OLD], a->d_rec, a->nrec,
+                                           a->d_dtrace, a->nsteps, a->d_counter, a->b->stream));
+    return enqueue_batch_adjoint_step(a);
+}
+
+template <class F> tsp_status batch_capture_pair(hipStream_t stream, hipGraphExec_t *slot, int *roles, F &&body) {
+    if (*slot) return TSP_OK;
+    int saved[4];
+    std::memcpy(saved, roles, sizeof(saved));
+    TSP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    tsp_status st = body();
+    if (st == TSP_OK) st = body();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    std::memcpy(roles, saved, sizeof(saved));
+    if (st != TSP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *slot = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return TSP_OK;
+}
+
+tsp_status batch_run_forward_range(tsp_batch_adjoint *a, int n0, int n1) {
+    tsp_batch *b = a->b;
+    int n = n0;
+    while (n < n1) {
+        if (n1 - n >= 2) {
+            const int parity = (int)(b->steps & 1);
+            tsp_status st =
+                batch_capture_pair(b->stream, &a->fgraph[parity], b->role, [&] { return enqueue_batch_fwd_gather(a); });
+            if (st != TSP_OK) return st;
+            TSP_HIP(hipGraphLaunch(a->fgraph[parity], b->stream));
+            batch_advance_clock(b);
+            batch_advance_clock(b);
+            n += 2;
+        } else {
+            tsp_status st = enqueue_batch_fwd_gather(a);
+            if (st != TSP_OK) return st;
+            batch_advance_clock(b);
+            n += 1;
+        }
+    }
+    return TSP_OK;
+}
+
+tsp_status batch_run_backward_range(tsp_batch_adjoint *a, int n1, int n0, int *bparity) {
+    int n = n1;
+    while (n > n0) {
+        if (n - n0 >= 2) {
+            tsp_status st = batch_capture_pair(a->b->stream, &a->bgraph[*bparity], a->arole,
+                                               [&] { return enqueue_batch_inject_adjoint(a); });
+            if (st != TSP_OK) return st;
+            TSP_HIP(hipGraphLaunch(a->bgraph[*bparity], a->b->stream));
+            n -= 2;
+        } else {
+            tsp_status st = enqueue_batch_inject_adjoint(a);
+            if (st != TSP_OK) return st;
+            *bparity ^= 1;
+            n -= 1;
+        }
+    }
+    return TSP_OK;
+}
+
+int batch_ckpt_slot_of_step(const tsp_batch_adjoint *a, int step) {
+    if (step == a->nsteps) return a->nck - 1;
+    return step % a->K == 0 ? step / a->K : -1;
+}
+
+tsp_status batch_save_checkpoint(tsp_batch_adjoint *a, int slot) {
+    tsp_batch *b = a->b;
+    for (int i = 0; i < b->count; ++i) {
+        const tsp_desc &d = b->d[i];
+        const Layout &L = b->L[i];
+        TSP_HIP(tsp::launch_pack(a->d_ckpt + slot * a->state_elems + a->state_off[i],
+                                 batch_origin(b, i, b->role[TSP_ROLE_OLD]), d.n[0], d.n[1], d.n[2],
+                                 system_ncomp(d.system), (int)L.sj, (int)L.sk, (int)L.sc, b->stream));
+    }
+    return TSP_OK;
+}
+
+// checkpoint `slot` (the state `step` steps into the forward pass) back into the batch
+tsp_status batch_restore_checkpoint(tsp_batch_adjoint *a, int slot, int step, const std::vector<double> &time) {
+    tsp_batch *b = a->b;
+    b->steps = a->step0 + step;
+    std::memcpy(b->role, (b->steps & 1) ? kOddRoles : kEvenRoles, sizeof(b->role));
+    b->time = time;
+    for (int i = 0; i < b->count; ++i) {
+        const tsp_desc &d = b->d[i];
+        const Layout &L = b->L[i];
+        TSP_HIP(tsp::launch_unpack(batch_origin(b, i, b->role[TSP_ROLE_OLD]),
+                                   a->d_ckpt + slot * a->state_elems + a->state_off[i], d.n[0], d.n[1], d.n[2],
+                                   system_ncomp(d.system), (int)L.sj, (int)L.sk, (int)L.sc, b->stream));
+    }
+    return TSP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+tsp_status tsp_batch_adjoint_create(tsp_batch *b, const int32_t *receivers, int32_t nrec, int32_t nsteps,
+                                    int32_t checkpoint_every, tsp_batch_adjoint **out) {
+    if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (nrec < 1 || !receivers) return fail(TSP_INVALID_ARG, "need at least one receiver");
+    if (nsteps < 1) return fail(TSP_INVALID_ARG, "nsteps must be >= 1");
+    if (checkpoint_every < 1) return fail(TSP_INVALID_ARG, "checkpoint_every must be >= 1");
+    std::vector<int> off((size_t)b->count * nrec);
+    for (int i = 0; i < b->count; ++i)
+        for (int r = 0; r < nrec; ++r) {
+            const int x = receivers[3 * r], y = receivers[3 * r + 1], z = receivers[3 * r + 2];
+            const tsp_desc &d = b->d[i];
+            if (x < 0 || x >= d.n[0] || y < 0 || y >= d.n[1] || z < 0 || z >= d.n[2])
+                return fail(TSP_INVALID_ARG, "receiver " + std::to_string(r) + " lies outside the valid box of instance " +
+                                                 std::to_string(i));
+            off[(size_t)i * nrec + r] = (int)(x + y * b->L[i].sj + z * b->L[i].sk); // component 0 (phi)
+        }
+    tsp_batch_adjoint *a = new (std::nothrow) tsp_batch_adjoint();
+    if (!a) return fail(TSP_OOM, "host allocation failed");
+    auto cleanup = [&](tsp_status e) {
+        tsp_batch_adjoint_destroy(a);
+        return e;
+    };
+    a->b = b;
+    a->nrec = nrec;
+    a->nsteps = nsteps;
+    a->K = checkpoint_every;
+    a->nck = nsteps / checkpoint_every + 1 + (nsteps % checkpoint_every != 0 ? 1 : 0);
+    for (int i = 0; i < b->count; ++i) {
+        a->state_off.push_back(a->state_elems);
+        a->state_elems += (int64_t)b->d[i].n[0] * b->d[i].n[1] * b->d[i].n[2] * system_ncomp(b->d[i].system);
+    }
+    const size_t trace_elems = (size_t)b->count * nsteps * nrec;
+    hipError_t e = hipMalloc((void **)&a->d_rec, sizeof(int) * off.size());
+    if (e == hipSuccess) e = hipMemcpy(a->d_rec, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_trace, sizeof(double) * trace_elems);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_dtrace, sizeof(double) * trace_elems);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_counter, sizeof(int) * b->count);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->d_ckpt, sizeof(double) * a->state_elems * a->nck);
+    // adjoint registers: the forward pool's layout, zero everywhere (ghosts stay zero:
+    // homogeneous boundary values)
+    if (e == hipSuccess) e = hipMalloc((void **)&a->apool, sizeof(double) * b->pool_elems);
+    if (e == hipSuccess) e = hipMemsetAsync(a->apool, 0, sizeof(double) * b->pool_elems, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(a->d_trace, 0, sizeof(double) * trace_elems, b->stream);
+    if (e == hipSuccess) {
+        a->h_ainst = b->h_inst;
+        for (int i = 0; i < b->count; ++i) {
+            tsp::BatchInstance &I = a->h_ainst[i];
+            for (int r = 0; r < 4; ++r) I.reg[r] = I.reg[r] - b->pool + a->apool;
+            for (int c = 0; c < TSP_MAX_COMP; ++c) I.target[c] = 0.0;
+        }
+        e = hipMalloc((void **)&a->d_ainst, sizeof(tsp::BatchInstance) * b->count);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(a->d_ainst, a->h_ainst.data(), sizeof(tsp::BatchInstance) * b->count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e == hipErrorOutOfMemory) return cleanup(fail(TSP_OOM, "adjoint buffers do not fit in device memory"));
+    if (e != hipSuccess) return cleanup(hip_fail(e, "batch adjoint allocation"));
+    *out = a;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_adjoint_destroy(tsp_batch_adjoint *a) {
+    if (!a) return TSP_OK;
+    if (a->b && a->b->stream) (void)hipStreamSynchronize(a->b->stream);
+    drop_batch_adjoint_graphs(a);
+    for (void *p : {(void *)a->d_rec, (void *)a->d_trace, (void *)a->d_dtrace, (void *)a->d_counter, (void *)a->d_ckpt,
+                    (void *)a->apool, (void *)a->d_ainst})
+        if (p) (void)hipFree(p);
+    delete a;
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_adjoint_forward(tsp_batch_adjoint *a) {
+    if (!a) return fail(TSP_INVALID_ARG, "adjoint is NULL");
+    tsp_batch *b = a->b;
+    tsp_status st = batch_adjoint_check(a);
+    if (st != TSP_OK) return st;
+    a->step0 = b->steps;
+    a->time0 = b->time;
+    if ((st = batch_save_checkpoint(a, 0)) != TSP_OK) return st;
+    TSP_HIP(tsp::launch_batch_set_counter(a->d_counter, b->count, 0, b->stream));
+    int n = 0;
+    while (n < a->nsteps && st == TSP_OK) {
+        const int e = std::min(a->nsteps, (n / a->K + 1) * a->K);
+        st = batch_run_forward_range(a, n, e);
+        n = e;
+        if (st == TSP_OK) {
+            const int slot = batch_ckpt_slot_of_step(a, n);
+            if (slot >= 0) st = batch_save_checkpoint(a, slot);
+        }
+    }
+    return st;
+}
+
+tsp_status tsp_batch_adjoint_trace(tsp_batch_adjoint *a, double *dst, int32_t on_device) {
+    if (!a || !dst) return fail(TSP_INVALID_ARG, "adjoint or destination is NULL");
+    TSP_HIP(hipStreamSynchronize(a->b->stream));
+    TSP_HIP(hipMemcpy(dst, a->d_trace, sizeof(double) * (size_t)a->b->count * a->nsteps * a->nrec,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_adjoint_backward(tsp_batch_adjoint *a, const double *dtrace, int32_t dtrace_on_device,
+                                      double *grad, int32_t grad_on_device, int32_t recompute) {
+    if (!a || !dtrace || !grad) return fail(TSP_INVALID_ARG, "adjoint, dtrace or grad is NULL");
+    tsp_batch *b = a->b;
+    tsp_status st = batch_adjoint_check(a);
+    if (st != TSP_OK) return st;
+    TSP_HIP(hipMemcpyAsync(a->d_dtrace, dtrace, sizeof(double) * (size_t)b->count * a->nsteps * a->nrec,
+                           dtrace_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, b->stream));
+    std::memcpy(a->arole, kEvenRoles, sizeof(a->arole));
+    for (int i = 0; i < b->count; ++i) // lambda = 0 in adjoint register 0 (ghosts stay zero)
+        TSP_HIP(tsp::launch_zero(a->apool + b->reg_off[4 * i], b->L[i].elems, b->stream));
+    int bparity = 0;
+    if (!recompute) {
+        TSP_HIP(tsp::launch_batch_set_counter(a->d_counter, b->count, a->nsteps - 1, b->stream));
+        st = batch_run_backward_range(a, a->nsteps, 0, &bparity);
+    } else {
+        // segments from the last to the first: restore the segment's checkpoint,
+        // recompute its forward states and trace, then reverse it
+        const std::vector<double> saved_time = b->time;
+        const int nseg = (a->nsteps + a->K - 1) / a->K;
+        std::vector<std::vector<double>> seg_time(nseg + 1, a->time0);
+        for (int m = 1; m <= nseg; ++m) { // every instance's clock at each segment start, accumulated as the steps do
+            seg_time[m] = seg_time[m - 1];
+            const int len = std::min(a->K, a->nsteps - (m - 1) * a->K);
+            for (int i = 0; i < b->count; ++i)
+                for (int q = 0; q < len; ++q) seg_time[m][i] += b->d[i].dt;
+        }
+        for (int m = nseg - 1; m >= 0 && st == TSP_OK; --m) {
+            const int n0 = m * a->K, n1 = std::min(a->nsteps, n0 + a->K);
+            st = batch_restore_checkpoint(a, m, n0, seg_time[m]);
+            if (st == TSP_OK) {
+                hipError_t e = tsp::launch_batch_set_counter(a->d_counter, b->count, n0, b->stream);
+                if (e != hipSuccess) st = hip_fail(e, "set counter");
+            }
+            if (st == TSP_OK) st = batch_run_forward_range(a, n0, n1);
+            if (st == TSP_OK) {
+                hipError_t e = tsp::launch_batch_set_counter(a->d_counter, b->count, n1 - 1, b->stream);
+                if (e != hipSuccess) st = hip_fail(e, "set counter");
+            }
+            if (st == TSP_OK) st = batch_run_backward_range(a, n1, n0, &bparity);
+        }
+        // leave the batch at the end of the forward pass
+        if (st == TSP_OK) st = batch_restore_checkpoint(a, a->nck - 1, a->nsteps, saved_time);
+    }
+    if (st != TSP_OK) return st;
+    double *dst = grad;
+    double *staging = nullptr;
+    if (!grad_on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, sizeof(double) * a->state_elems));
+        dst = staging;
+    }
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < b->count && e == hipSuccess; ++i) {
+        const tsp_desc &d = b->d[i];
+        const Layout &L = b->L[i];
+        e = tsp::launch_pack(dst + a->state_off[i], a->h_ainst[i].reg[a->arole[TSP_ROLE_OLD]], d.n[0], d.n[1], d.n[2],
+                             system_ncomp(d.system), (int)L.sj, (int)L.sk, (int)L.sc, b->stream);
+    }
+    if (e == hipSuccess && staging)
+        e = hipMemcpyAsync(grad, staging, sizeof(double) * a->state_elems, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) return hip_fail(e, "batch adjoint gradient");
     return TSP_OK;
 }
 

@@ -72,9 +72,12 @@ __device__ __forceinline__ int xcd_item(int w, int nwork) {
 // group's work items. RS > 0: the halo-ring tile body. Items are 64 x TY
 // tiles or, flagged narrow, 32 x 2*TY tiles (same thread count).
 // SM: compile-time stage kind (forward stage 0 or 3) or -1 (from the arguments).
-template <class Sys, int N, int R, int TY, int RS, int SM = -1> struct BatchedStage {
-    using FS = FusedStage<Sys, N, R, TY, false, RS, true, 64, SM>;
-    using NFS = FusedStage<Sys, N, R, 2 * TY, false, RS, true, 32, SM>;
+// ADJ: reverse stage j = b.stage of the discrete adjoint (Sys the transposed
+// system), with the coefficients the single-box adjoint forms from the
+// instance's dt.
+template <class Sys, int N, int R, int TY, int RS, int SM = -1, bool ADJ = false> struct BatchedStage {
+    using FS = FusedStage<Sys, N, R, TY, ADJ, RS, true, 64, SM>;
+    using NFS = FusedStage<Sys, N, R, 2 * TY, ADJ, RS, true, 32, SM>;
     static constexpr int LDS_TOTAL = FS::LDS_TOTAL > NFS::LDS_TOTAL ? FS::LDS_TOTAL : NFS::LDS_TOTAL;
     static constexpr bool FITS = (size_t)LDS_TOTAL * sizeof(double) <= 144 * 1024;
 
@@ -113,10 +116,19 @@ template <class Sys, int N, int R, int TY, int RS, int SM = -1> struct BatchedSt
             // integrators/SingleLevelRK4Integrator.hpp:71-74,119-121 (the same
             // products the host forms for a single box)
             const int s = b.stage;
-            a.b = s == kEuler ? I.dt : I.dt * ((s == 0 || s == 3) ? 1.0 / 6.0 : 1.0 / 3.0);
-            a.a_next = I.dt * (s == 2 ? 1.0 : 0.5);
-            a.c_init_argn = 0.0;
-            a.src_scale = 1.0;
+            if constexpr (ADJ) {
+                // reverse stage j: lambda'-scale b_3 on the first, next argument
+                // b_{j-1} lambda' + a_j nu (b = dt*{1/6,1/3,1/3,1/6}, a = dt*{0,1/2,1/2,1})
+                a.b = 0.0;
+                a.a_next = s > 0 ? I.dt * (s == 3 ? 1.0 : 0.5) : 0.0;
+                a.src_scale = s == 3 ? I.dt * (1.0 / 6.0) : 1.0;
+                a.c_init_argn = s > 0 ? I.dt * (s == 1 ? 1.0 / 6.0 : 1.0 / 3.0) : 0.0;
+            } else {
+                a.b = s == kEuler ? I.dt : I.dt * ((s == 0 || s == 3) ? 1.0 / 6.0 : 1.0 / 3.0);
+                a.a_next = I.dt * (s == 2 ? 1.0 : 0.5);
+                a.c_init_argn = 0.0;
+                a.src_scale = 1.0;
+            }
             a.gf1[0] = I.gf1[0];
             a.gf1[1] = I.gf1[1];
             a.gf1[2] = I.gf1[2];
@@ -146,39 +158,39 @@ template <class Sys, int N, int R, int TY, int RS, int SM = -1> struct BatchedSt
 // The instance table and work list are separate restrict-qualified arguments:
 // nothing the kernel stores can alias them, so their uniform loads can be
 // scalar loads.
-template <class Sys, int N, int R, int TY, int RS = 0, int SM = -1>
+template <class Sys, int N, int R, int TY, int RS = 0, int SM = -1, bool ADJ = false>
 __global__ __launch_bounds__(64 * TY) void batched_stage_kernel(const BatchInstance *__restrict__ inst,
                                                                 const BatchWork *__restrict__ work,
                                                                 const BatchArgs b) {
-    __shared__ double lds[BatchedStage<Sys, N, R, TY, RS, SM>::LDS_TOTAL];
+    __shared__ double lds[BatchedStage<Sys, N, R, TY, RS, SM, ADJ>::LDS_TOTAL];
     extern __shared__ double sig_lds[];
-    BatchedStage<Sys, N, R, TY, RS, SM>::run(inst, work, b, lds, sig_lds);
+    BatchedStage<Sys, N, R, TY, RS, SM, ADJ>::run(inst, work, b, lds, sig_lds);
 }
 
 // one tile height: the halo-ring kernel when asked for and its LDS fits
-template <class Sys, int N, int R, int TY, int SM>
+template <class Sys, int N, int R, int TY, int SM, bool ADJ = false>
 hipError_t launch_batched_tile(const BatchArgs &b, int grid, size_t shm, int halo_ring, hipStream_t stream) {
     constexpr int RS = ring_slots<TY, FusedStage<Sys, N, R, TY>::H>();
-    using Ring = BatchedStage<Sys, N, R, TY, RS, SM>;
+    using Ring = BatchedStage<Sys, N, R, TY, RS, SM, ADJ>;
     if constexpr (RS > 0 && Ring::FITS) {
         if (halo_ring && sizeof(double) * Ring::LDS_TOTAL + shm <= max_block_lds()) {
-            hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS, SM>), dim3(grid), dim3(64 * TY), shm, stream,
-                               b.inst, b.work, b);
+            hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, RS, SM, ADJ>), dim3(grid), dim3(64 * TY), shm,
+                               stream, b.inst, b.work, b);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, 0, SM>), dim3(grid), dim3(64 * TY), shm, stream, b.inst,
-                       b.work, b);
+    hipLaunchKernelGGL((batched_stage_kernel<Sys, N, R, TY, 0, SM, ADJ>), dim3(grid), dim3(64 * TY), shm, stream,
+                       b.inst, b.work, b);
     return hipGetLastError();
 }
 
 // `max_abl_T`: the thickest absorbing layer in the group (sizes the LDS rate table)
-template <class Sys, int N, int R, int SM>
+template <class Sys, int N, int R, int SM, bool ADJ = false>
 hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring, hipStream_t stream) {
     const size_t shm = sigma_lds_bytes(max_abl_T);
     switch (tile_y) {
-    case 8: return launch_batched_tile<Sys, N, R, 8, SM>(b, grid, shm, halo_ring, stream);
-    case 16: return launch_batched_tile<Sys, N, R, 16, SM>(b, grid, shm, halo_ring, stream);
+    case 8: return launch_batched_tile<Sys, N, R, 8, SM, ADJ>(b, grid, shm, halo_ring, stream);
+    case 16: return launch_batched_tile<Sys, N, R, 16, SM, ADJ>(b, grid, shm, halo_ring, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -204,5 +216,18 @@ hipError_t launch_batched_phipi_s3(int variant, const BatchArgs &b, int tile_y, 
 // launch one variant group of the (phi, Pi) batch
 hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                 hipStream_t stream);
+// one reverse stage (b.stage = j) of the discrete adjoint for one variant group
+// (transposed system on the adjoint registers; batch_adjoint.hip)
+hipError_t launch_batched_phipi_adjoint(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                        int halo_ring, hipStream_t stream);
+
+// Receiver traces of a batched adjoint: block i serves instance i, reading
+// register `role`; trace/dtrace rows are [instance][row][receiver] and every
+// instance keeps its own row counter (counter[i]), advanced by its block.
+hipError_t launch_batch_gather_trace(const BatchInstance *inst, int count, int role, const int *rec_off, int nrec,
+                                     double *trace, int nrows, int *counter, hipStream_t stream);
+hipError_t launch_batch_inject_trace(const BatchInstance *inst, int count, int role, const int *rec_off, int nrec,
+                                     const double *dtrace, int nrows, int *counter, hipStream_t stream);
+hipError_t launch_batch_set_counter(int *counter, int count, int v, hipStream_t stream);
 
 } // namespace tsp

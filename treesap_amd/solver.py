@@ -196,6 +196,10 @@ class Batch:
         return len(self.cases)
 
     @property
+    def handle(self):
+        return self._h
+
+    @property
     def cells(self) -> int:
         n = C.c_int32()
         c = C.c_int64()
@@ -263,6 +267,10 @@ class Batch:
 
     def graph_rk4_steps(self, nsteps: int) -> None:
         _lib.check(_lib.lib().tsp_batch_graph_rk4_steps(self._h, int(nsteps)), "tsp_batch_graph_rk4_steps")
+
+    def graph_prepare(self) -> None:
+        """Capture the two-step graphs of both step parities (nothing is launched)."""
+        _lib.check(_lib.lib().tsp_batch_graph_prepare(self._h), "tsp_batch_graph_prepare")
 
     def time_rk4(self, nsteps: int, use_graph: bool = True) -> float:
         ms = C.c_double()
