@@ -2,7 +2,11 @@
 // lives in device memory behind the C ABI (SingleLevelIntegrator.hpp:22-187).
 //
 // Construction maps the model onto a device solver (device/model_map.hpp) and
-// allocates its registers; `block_size` is accepted for source compatibility.
+// allocates its registers. Right-hand sides the library carries (registered, or
+// mapped by system_traits) run on its tiled stage kernels; any other functor
+// runs on a stage kernel compiled from it (device/generic.hpp), which needs
+// this program to be compiled by a HIP compiler. `block_size` is accepted for
+// source compatibility.
 // The reference splits the level into boxes of at most block_size cells, and
 // its results do not depend on that split; here the level is one box, and
 // the ghost exchange between boxes never arises.
@@ -19,6 +23,7 @@
 #pragma once
 
 #include "Integrator.hpp"
+#include "device/generic_runner.hpp"
 #include "device/model_map.hpp"
 #include "device/solver.hpp"
 #include "util/PlotExporter.hpp"
@@ -43,10 +48,15 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     using Base::time;
 
   public:
+    using RHSType = typename PDEModelType::rhs_t;
+    // whether the library's tiled kernels run this model (else the generic kernel)
+    static constexpr bool library_kernels = device::runs_library_kernels<PDEModelType>;
+
     SingleLevelIntegrator(PDEModelType model_, const amrex::IntVect &block_size)
         : Base(std::move(model_)), solver(device::make_desc(model)) {
         for (int a = 0; a < 3; ++a)
             if (block_size[a] < 1) throw std::invalid_argument("treesap: block_size entries must be positive");
+        if constexpr (!library_kernels) generic.emplace(model.get_rhs(), model.get_geometry().data());
     }
     ~SingleLevelIntegrator() override = default;
 
@@ -134,6 +144,7 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     }
 
     mutable device::Solver solver;
+    std::optional<device::GenericRunner<RHSType>> generic; // models on the generic kernel
 
   private:
     std::optional<PlotExporter> exporter;

@@ -8,8 +8,10 @@
 // written against TreESaP therefore names these types; this header supplies
 // them with the same member functions and semantics for a single-level,
 // single-box, cell-centred 3-D Cartesian domain, so such code compiles
-// unchanged. The functors run on the host only (the device work happens in the
-// library behind include/treesap_hip.h), so the function-space macros are empty.
+// unchanged. Compiled by a HIP compiler, the function-space macros mark
+// functions for host and device (functors the library does not carry run in a
+// kernel compiled from them, device/generic.hpp); compiled by a host compiler
+// they are empty.
 //
 // Semantics kept (AMReX's documented behaviour):
 //   - Box is a cell-centred index box [smallEnd, bigEnd], inclusive;
@@ -27,9 +29,16 @@
 #include <type_traits>
 #include <vector>
 
+#if defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define AMREX_GPU_HOST_DEVICE __host__ __device__
+#define AMREX_GPU_DEVICE __device__
+#define AMREX_GPU_HOST __host__
+#else
 #define AMREX_GPU_HOST_DEVICE
 #define AMREX_GPU_DEVICE
 #define AMREX_GPU_HOST
+#endif
 #define AMREX_FORCE_INLINE inline
 #define AMREX_SPACEDIM 3
 #define AMREX_D_DECL(a, b, c) a, b, c
@@ -143,6 +152,9 @@ template <class T> struct Array4 {
     constexpr Array4(T *ptr, Dim3 b, Dim3 e, int nc)
         : p(ptr), jstride((std::int64_t)(e.x - b.x)), kstride((std::int64_t)(e.x - b.x) * (e.y - b.y)),
           nstride((std::int64_t)(e.x - b.x) * (e.y - b.y) * (e.z - b.z)), begin(b), end(e), ncomp(nc) {}
+    // a view with explicit strides (padded rows), p pointing at element (b.x, b.y, b.z, 0)
+    constexpr Array4(T *ptr, std::int64_t js, std::int64_t ks, std::int64_t ns, Dim3 b, Dim3 e, int nc)
+        : p(ptr), jstride(js), kstride(ks), nstride(ns), begin(b), end(e), ncomp(nc) {}
     // a read-only view of a writable one
     template <class U>
         requires(std::is_same_v<const U, T> && !std::is_same_v<U, T>)
@@ -190,13 +202,13 @@ struct GeometryData {
     int is_periodic[3];
     int coord;
 
-    Real CellSize(int d) const noexcept { return dx[d]; }
-    const Real *CellSize() const noexcept { return dx; }
-    Real ProbLo(int d) const noexcept { return prob_domain_lo[d]; }
-    Real ProbHi(int d) const noexcept { return prob_domain_hi[d]; }
-    const Box &Domain() const noexcept { return domain; }
-    int isPeriodic(int d) const noexcept { return is_periodic[d]; }
-    int Coord() const noexcept { return coord; }
+    AMREX_GPU_HOST_DEVICE Real CellSize(int d) const noexcept { return dx[d]; }
+    AMREX_GPU_HOST_DEVICE const Real *CellSize() const noexcept { return dx; }
+    AMREX_GPU_HOST_DEVICE Real ProbLo(int d) const noexcept { return prob_domain_lo[d]; }
+    AMREX_GPU_HOST_DEVICE Real ProbHi(int d) const noexcept { return prob_domain_hi[d]; }
+    AMREX_GPU_HOST_DEVICE const Box &Domain() const noexcept { return domain; }
+    AMREX_GPU_HOST_DEVICE int isPeriodic(int d) const noexcept { return is_periodic[d]; }
+    AMREX_GPU_HOST_DEVICE int Coord() const noexcept { return coord; }
 };
 
 class Geometry {

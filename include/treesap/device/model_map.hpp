@@ -2,26 +2,29 @@
 // (tsp_desc), i.e. decides which device kernel runs the model.
 //
 // A model's right-hand side is a compile-time functor (RHSConcept,
-// pde-model/PDEModelConcepts.hpp:44-64). The device library carries kernels
-// for the systems of systems/registered_rhs.hpp, each with derivative
-// half-width n = 1..4, optionally wrapped as ABLDecorator<KODecorator<user>>
-// (the nesting ABLPDEModel produces when handed a KODecorator). The mapping:
+// pde-model/PDEModelConcepts.hpp:44-64). Two device paths:
 //
-//   decorators   unwrapped at compile time: KODecorator<R, r> gives ko_order r
-//                and its strength, ABLDecorator<R, P> its thickness, maximum,
-//                targets and profile. Other nestings (KO outside ABL, repeated
-//                decorators) are rejected at compile time: their rounding
-//                differs from the kernels'.
-//   profile      must equal s, s*s or s*s*s on every argument the layer uses
-//                (checked on the host, bit for bit).
-//   user RHS     either named through a specialisation of system_traits, or
-//                recognised: the functor is evaluated on the host on a seeded
-//                random field at 4^3 cells and two times, every component in
-//                order with one Workspace per cell, and compared with each
-//                registered functor of the same ncomp and every n <= ngrow
-//                (NonLinear uses the functor's `gamma` member). A match within
-//                1e-12 of the component's largest magnitude selects that
-//                system; no match throws.
+//   library kernels  for the functors of systems/registered_rhs.hpp and for any
+//                    functor a system_traits specialisation maps onto one of
+//                    them, optionally wrapped as ABLDecorator<KODecorator<user>>
+//                    (the nesting ABLPDEModel produces when handed a
+//                    KODecorator): the tiled stage kernels of the library.
+//                    Decorators are unwrapped at compile time (KODecorator<R, r>
+//                    gives ko_order r and its strength, ABLDecorator<R, P> its
+//                    thickness, maximum, targets and profile; other nestings do
+//                    not compile). The profile must equal s, s*s or s*s*s on every
+//                    argument the layer uses. A user's system_traits mapping is
+//                    verified when the model is mapped: the functor and the
+//                    registered one are evaluated on seeded random fields at
+//                    cells spread over the whole domain (corners, faces, the
+//                    absorbing layer, the bulk, random cells) at several times,
+//                    every component in order with one Workspace per cell, and
+//                    must agree bit for bit; otherwise the mapping throws.
+//   generic kernel   every other functor (TSP_SYS_CUSTOM): the decorated functor
+//                    itself is compiled into the stage kernel of
+//                    device/generic.hpp; the library fills ghosts and keeps the
+//                    registers and the clock.
+//
 //   BC           DirichletBC -> TSP_BC_DIRICHLET with its values, NoOpBC ->
 //                TSP_BC_NOOP; other functors are rejected at compile time. With
 //                both an ABL and a DirichletBC the two value sets must agree
@@ -55,8 +58,21 @@ namespace treesap {
 //       static constexpr int stencil_n = g;                  // derivative half-width
 //       static double gamma(const MyRHS<g> &) { return 0.0; } // NonLinear coefficient
 //   };
-// Types without one are recognised by probing (see above).
+// A user's mapping is verified when the model is mapped (see above); types
+// without one run on the generic kernel (device/generic.hpp).
 template <class RHS> struct system_traits {};
+
+namespace device {
+// the library's own functors: exact by construction, not probed
+template <class RHS> constexpr bool is_builtin = false;
+template <int n, int g> constexpr bool is_builtin<systems::PhiPiWaveRHS<n, g>> = true;
+template <int n, int g> constexpr bool is_builtin<systems::PhiUWaveRHS<n, g>> = true;
+template <int n, int g> constexpr bool is_builtin<systems::NonLinearWaveRHS<n, g>> = true;
+template <int n, int g> constexpr bool is_builtin<systems::LinearADMRHS<n, g>> = true;
+
+// whether the library's kernels run a functor (directly or through a system_traits mapping)
+template <class RHS> constexpr bool is_mapped = requires { system_traits<RHS>::system; };
+} // namespace device
 
 template <int n, int g> struct system_traits<systems::PhiPiWaveRHS<n, g>> {
     static constexpr int system = TSP_SYS_WAVE_PHIPI, stencil_n = n;
@@ -101,87 +117,118 @@ struct SplitMix64 {
     double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0; }
 };
 
-// Evaluate every component of `f` in order, one Workspace per cell, at the
-// P^3 cells of a random field with H ghost layers.
-template <class F, int H, int P>
-std::vector<double> probe_values(const F &f, const std::vector<double> &field, const amrex::GeometryData &g, double t) {
-    constexpr int nc = F::ncomp;
-    const amrex::Array4<const double> arr(field.data(), {-H, -H, -H}, {P + H, P + H, P + H}, nc);
-    std::vector<double> out((std::size_t)P * P * P * nc);
-    for (int k = 0; k < P; ++k)
-        for (int j = 0; j < P; ++j)
-            for (int i = 0; i < P; ++i) {
-                const amrex::IntVect iv(i, j, k);
-                typename F::Workspace ws{};
-                treesap::static_for<nc>([&]<int c>() {
-                    out[(((std::size_t)c * P + k) * P + j) * P + i] = f.template operator()<c>(iv, t, arr, g, ws);
-                });
-            }
+// Cells of the domain at which a mapping is verified: the 8 corners, the 6
+// face centres, the centre, a cell `inset` cells in from every corner (inside
+// an absorbing layer of that depth) and 16 seeded random cells.
+inline std::vector<amrex::IntVect> probe_cells(const amrex::Box &dom, int inset) {
+    std::vector<amrex::IntVect> out;
+    const amrex::IntVect lo = dom.smallEnd(), hi = dom.bigEnd();
+    for (int c = 0; c < 8; ++c)
+        out.push_back({(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]});
+    const amrex::IntVect mid{(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+    out.push_back(mid);
+    for (int a = 0; a < 3; ++a) {
+        amrex::IntVect f = mid;
+        f[a] = lo[a];
+        out.push_back(f);
+        f[a] = hi[a];
+        out.push_back(f);
+    }
+    for (int c = 0; c < 8; ++c) {
+        amrex::IntVect v;
+        for (int a = 0; a < 3; ++a) {
+            const int d = std::min(inset, (hi[a] - lo[a]) / 2);
+            v[a] = ((c >> a) & 1) ? hi[a] - d : lo[a] + d;
+        }
+        out.push_back(v);
+    }
+    SplitMix64 rng{0x70726f6265636c6cull};
+    for (int q = 0; q < 16; ++q) {
+        amrex::IntVect v;
+        for (int a = 0; a < 3; ++a) v[a] = lo[a] + (int)(rng.next() % (std::uint64_t)(hi[a] - lo[a] + 1));
+        out.push_back(v);
+    }
     return out;
 }
 
-template <class A, class B> bool probe_equal(const A &a, const B &b, const amrex::GeometryData &g) {
+// Whether functors `a` and `b` give bit-identical values (every component in
+// order, one Workspace per cell) on seeded random fields around every probe
+// cell, at several times. Returns a description of the first difference.
+template <class A, class B>
+bool probe_equal(const A &a, const B &b, const amrex::GeometryData &g, int inset, std::string *why) {
     static_assert(A::ncomp == B::ncomp);
     constexpr int nc = A::ncomp;
     constexpr int H = A::ngrow > B::ngrow ? A::ngrow : B::ngrow;
-    constexpr int P = 4;
-    constexpr int W = P + 2 * H;
+    constexpr int W = 2 * H + 1;
     std::vector<double> field((std::size_t)W * W * W * nc);
     SplitMix64 rng{0x7265636f676e6973ull};
-    for (double &v : field) v = rng.uniform();
-    for (double t : {0.375, 1.90625}) {
-        const auto va = probe_values<A, H, P>(a, field, g, t);
-        const auto vb = probe_values<B, H, P>(b, field, g, t);
-        const std::size_t per = (std::size_t)P * P * P;
-        for (int c = 0; c < nc; ++c) {
-            double scale = 0.0;
-            for (std::size_t q = 0; q < per; ++q) scale = std::fmax(scale, std::fabs(vb[c * per + q]));
-            for (std::size_t q = 0; q < per; ++q) {
-                const double x = va[c * per + q], y = vb[c * per + q];
-                if (!std::isfinite(x) || !(std::fabs(x - y) <= 1e-12 * scale)) return false;
+    for (const amrex::IntVect &cell : probe_cells(g.Domain(), inset)) {
+        for (double &v : field) v = rng.uniform();
+        const amrex::Array4<const double> arr(field.data(), {cell[0] - H, cell[1] - H, cell[2] - H},
+                                              {cell[0] + H + 1, cell[1] + H + 1, cell[2] + H + 1}, nc);
+        for (double t : {0.0, 0.375, 1.90625}) {
+            typename A::Workspace wa{};
+            typename B::Workspace wb{};
+            bool same = true;
+            int bad = -1;
+            double va = 0.0, vb = 0.0;
+            treesap::static_for<nc>([&]<int c>() {
+                const double x = a.template operator()<c>(cell, t, arr, g, wa);
+                const double y = b.template operator()<c>(cell, t, arr, g, wb);
+                if (same && std::memcmp(&x, &y, sizeof(double)) != 0) {
+                    same = false;
+                    bad = c;
+                    va = x;
+                    vb = y;
+                }
+            });
+            if (!same) {
+                if (why)
+                    *why = "component " + std::to_string(bad) + " at cell (" + std::to_string(cell[0]) + "," +
+                           std::to_string(cell[1]) + "," + std::to_string(cell[2]) + "), t=" + std::to_string(t) +
+                           ": " + std::to_string(va) + " vs " + std::to_string(vb);
+                return false;
             }
         }
     }
     return true;
 }
 
-template <class U, int n> Recognised try_half_width(const U &u, const amrex::GeometryData &g) {
-    constexpr int G = U::ngrow;
-    if constexpr (U::ncomp == 2) {
-        if constexpr (requires { { u.gamma } -> std::convertible_to<double>; }) {
-            const systems::NonLinearWaveRHS<n, G> cand{static_cast<double>(u.gamma)};
-            if (probe_equal(u, cand, g)) return {TSP_SYS_NONLINEAR, n, cand.gamma};
-        }
-        if (probe_equal(u, systems::PhiPiWaveRHS<n, G>{}, g)) return {TSP_SYS_WAVE_PHIPI, n, 0.0};
-    } else if constexpr (U::ncomp == 4) {
-        if (probe_equal(u, systems::PhiUWaveRHS<n, G>{}, g)) return {TSP_SYS_WAVE_PHIU, n, 0.0};
-    } else if constexpr (U::ncomp == 12) {
-        if (probe_equal(u, systems::LinearADMRHS<n, G>{}, g)) return {TSP_SYS_TENSOR_ADM, n, 0.0};
-    }
-    return {};
-}
+template <int sys, int n, int g> struct registered_functor;
+template <int n, int g> struct registered_functor<TSP_SYS_WAVE_PHIPI, n, g> {
+    template <class U> static systems::PhiPiWaveRHS<n, g> make(double) { return {}; }
+};
+template <int n, int g> struct registered_functor<TSP_SYS_WAVE_PHIU, n, g> {
+    template <class U> static systems::PhiUWaveRHS<n, g> make(double) { return {}; }
+};
+template <int n, int g> struct registered_functor<TSP_SYS_NONLINEAR, n, g> {
+    template <class U> static systems::NonLinearWaveRHS<n, g> make(double gamma) { return {gamma}; }
+};
+template <int n, int g> struct registered_functor<TSP_SYS_TENSOR_ADM, n, g> {
+    template <class U> static systems::LinearADMRHS<n, g> make(double) { return {}; }
+};
 
 } // namespace detail
 
-// Which device system evaluates the undecorated right-hand side `u`.
-template <class U> Recognised recognise(const U &u, const amrex::GeometryData &g) {
-    if constexpr (requires { system_traits<U>::system; }) {
-        return {system_traits<U>::system, system_traits<U>::stencil_n, system_traits<U>::gamma(u)};
-    } else {
-        Recognised r;
-        treesap::static_for<1, 5>([&]<int n>() {
-            if constexpr (n <= U::ngrow)
-                if (r.system < 0) r = detail::try_half_width<U, n>(u, g);
-        });
-        if (r.system < 0)
-            throw std::invalid_argument(
-                std::string("treesap: the right-hand side ") + typeid(U).name() + " (ncomp " + std::to_string(U::ncomp) +
-                ", ngrow " + std::to_string(U::ngrow) +
-                ") matches none of the systems the device library carries: (phi,Pi) wave, (phi,u) wave, "
-                "NonLinear (phi,pi), linearised ADM (h,K), derivative half-width 1..4. Write it with "
-                "systems/registered_rhs.hpp's arithmetic or specialise treesap::system_traits for it.");
-        return r;
+// Which library system evaluates the undecorated right-hand side `u` (which
+// must be mapped: registered, or named by a system_traits specialisation); a
+// user's mapping is verified by probing, and a mismatch throws.
+template <class U> Recognised recognise(const U &u, const amrex::GeometryData &g, int inset) {
+    static_assert(is_mapped<U>, "recognise() is for mapped functors");
+    using T = system_traits<U>;
+    const Recognised r{T::system, T::stencil_n, T::gamma(u)};
+    if constexpr (!is_builtin<U>) {
+        static_assert(T::stencil_n >= 1 && T::stencil_n <= 4, "system_traits: stencil_n must be in 1..4");
+        const auto ref = detail::registered_functor<T::system, T::stencil_n, U::ngrow>::template make<U>(r.gamma);
+        static_assert(decltype(ref)::ncomp == U::ncomp, "system_traits: the mapped system has another ncomp");
+        std::string why;
+        if (!detail::probe_equal(u, ref, g, inset, &why))
+            throw std::invalid_argument(std::string("treesap: system_traits maps the right-hand side ") + typeid(U).name() +
+                                        " onto library system " + std::to_string(T::system) +
+                                        " (stencil_n " + std::to_string(T::stencil_n) + "), but their values differ: " +
+                                        why);
     }
+    return r;
 }
 
 // ---- decorators ------------------------------------------------------------
@@ -264,17 +311,14 @@ template <int nc, int ng> struct bc_map<DirichletBC<nc, ng>> {
 
 // ---- the descriptor --------------------------------------------------------
 
+// Whether a model runs on the library's kernels (else on the generic kernel).
+template <class Model> constexpr bool runs_library_kernels = is_mapped<typename decorators<typename Model::rhs_t>::user_t>;
+
 template <class Model> tsp_desc make_desc(const Model &m) {
     static_assert(Model::ncomp <= TSP_MAX_COMP, "treesap: at most 12 components");
-    using Dec = decorators<typename Model::rhs_t>;
     const amrex::Geometry &geom = m.get_geometry();
-
     tsp_desc d;
     std::memset(&d, 0, sizeof(d));
-    const Recognised rec = recognise(Dec::user(m.get_rhs()), geom.data());
-    d.system = rec.system;
-    d.stencil_n = rec.stencil_n;
-    d.gamma = rec.gamma;
     d.ghost = Model::ngrow;
     for (int a = 0; a < 3; ++a) {
         d.n[a] = geom.Domain().length(a);
@@ -283,8 +327,20 @@ template <class Model> tsp_desc make_desc(const Model &m) {
         d.periodic[a] = geom.isPeriodic(a) ? 1 : 0;
     }
     d.abl_profile = TSP_PROFILE_QUADRATIC;
-    Dec::fill(m.get_rhs(), d);
-    bc_map<typename Model::bc_t>::fill(m.get_bc(), d, Dec::has_abl);
+    if constexpr (runs_library_kernels<Model>) {
+        using Dec = decorators<typename Model::rhs_t>;
+        Dec::fill(m.get_rhs(), d);
+        const Recognised rec = recognise(Dec::user(m.get_rhs()), geom.data(), d.abl_thickness > 0 ? d.abl_thickness / 2 : 2);
+        d.system = rec.system;
+        d.stencil_n = rec.stencil_n;
+        d.gamma = rec.gamma;
+        bc_map<typename Model::bc_t>::fill(m.get_bc(), d, Dec::has_abl);
+    } else {
+        // the generic kernel evaluates the decorated functor itself
+        d.system = TSP_SYS_CUSTOM;
+        d.ncomp = Model::ncomp;
+        bc_map<typename Model::bc_t>::fill(m.get_bc(), d, false);
+    }
     d.dt = m.dt_hint();
     return d;
 }

@@ -11,6 +11,8 @@ template <typename PDEModelType>
     requires PDEModelConcept<PDEModelType>
 class SingleLevelEulerIntegrator : public SingleLevelIntegrator<PDEModelType> {
     using Base = SingleLevelIntegrator<PDEModelType>;
+    using Base::generic;
+    using Base::library_kernels;
     using Base::require_model_dt;
     using Base::solver;
 
@@ -20,12 +22,16 @@ class SingleLevelEulerIntegrator : public SingleLevelIntegrator<PDEModelType> {
 
     amrex::Real computeNewState(amrex::Real delta_time) override {
         require_model_dt(delta_time);
-        solver.euler_steps(1);
+        if constexpr (library_kernels) solver.euler_steps(1);
+        else generic->euler_steps(solver, 1);
         return delta_time;
     }
 
   protected:
-    void device_steps(std::int64_t n, bool) override { solver.euler_steps(n); }
+    void device_steps(std::int64_t n, bool) override {
+        if constexpr (library_kernels) solver.euler_steps(n);
+        else generic->euler_steps(solver, n);
+    }
 };
 
 } // namespace treesap

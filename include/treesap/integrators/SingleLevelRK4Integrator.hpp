@@ -16,6 +16,8 @@ template <typename PDEModelType>
     requires PDEModelConcept<PDEModelType>
 class SingleLevelRK4Integrator : public SingleLevelIntegrator<PDEModelType> {
     using Base = SingleLevelIntegrator<PDEModelType>;
+    using Base::generic;
+    using Base::library_kernels;
     using Base::require_model_dt;
     using Base::solver;
 
@@ -26,13 +28,15 @@ class SingleLevelRK4Integrator : public SingleLevelIntegrator<PDEModelType> {
     // one RK4 step of the whole level
     amrex::Real computeNewState(amrex::Real delta_time) override {
         require_model_dt(delta_time);
-        solver.rk4_steps(1);
+        if constexpr (library_kernels) solver.rk4_steps(1);
+        else generic->rk4_steps(solver, 1);
         return delta_time;
     }
 
   protected:
     void device_steps(std::int64_t n, bool graph) override {
-        if (graph) solver.graph_rk4_steps(n);
+        if constexpr (!library_kernels) generic->rk4_steps(solver, n);
+        else if (graph) solver.graph_rk4_steps(n);
         else solver.rk4_steps(n);
     }
 };

@@ -14,7 +14,7 @@ template <int ncomp_, int ngrow_> struct DirichletBC {
 
     amrex::GpuArray<amrex::Real, ncomp> boundary_values{};
 
-    void operator()(const amrex::IntVect &iv, const amrex::Array4<amrex::Real> &dest, int, int,
+    AMREX_GPU_HOST_DEVICE void operator()(const amrex::IntVect &iv, const amrex::Array4<amrex::Real> &dest, int, int,
                     const amrex::GeometryData &geom, amrex::Real, const amrex::BCRec *, int, int) const {
         if (geom.Domain().contains(iv)) return;
         for (int c = 0; c < ncomp; ++c) dest(iv, c) = boundary_values[c];

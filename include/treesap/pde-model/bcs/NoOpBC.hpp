@@ -12,7 +12,7 @@ template <int ncomp_, int ngrow_> struct NoOpBC {
     static constexpr int ncomp = ncomp_;
     static constexpr int ngrow = ngrow_;
 
-    void operator()(const amrex::IntVect &, const amrex::Array4<amrex::Real> &, int, int, const amrex::GeometryData &,
+    AMREX_GPU_HOST_DEVICE void operator()(const amrex::IntVect &, const amrex::Array4<amrex::Real> &, int, int, const amrex::GeometryData &,
                     amrex::Real, const amrex::BCRec *, int, int) const {}
 };
 

@@ -50,13 +50,13 @@ struct ABLDecorator {
         : rhs(std::move(rhs_)), data(std::move(data_)), bulk(amrex::grow(geom.Domain(), -data.thickness)) {}
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real time, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real time, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &workspace) const {
         return rhs.template operator()<comp>(idx, time, arr, geom, workspace) -
                sigma(idx) * (arr(idx, comp) - data.target_values[comp]);
     }
 
-    amrex::Real sigma(const amrex::IntVect &idx) const {
+    AMREX_GPU_HOST_DEVICE amrex::Real sigma(const amrex::IntVect &idx) const {
         if (bulk.contains(idx)) return 0.0;
         int d2 = 0;
         for (int a = 0; a < 3; ++a) {

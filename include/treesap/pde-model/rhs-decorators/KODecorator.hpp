@@ -25,7 +25,7 @@ struct KODecorator {
     using inner_rhs_t = RHSType;
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real time, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real time, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &workspace) const {
         amrex::Real ko = 0.0;
         ko += stencil::kreiss_oliger<order, 0>(arr, comp, idx, geom);

@@ -12,7 +12,7 @@ template <int ncomp_, int ngrow_> struct NoOpRHS {
     struct Workspace {};
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect, amrex::Real, const amrex::Array4<const amrex::Real> &,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect, amrex::Real, const amrex::Array4<const amrex::Real> &,
                            const amrex::GeometryData &, Workspace &) const {
         return 0.0;
     }

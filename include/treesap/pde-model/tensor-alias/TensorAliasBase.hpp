@@ -97,19 +97,19 @@ struct TensorAliasBase {
         return Derived::restore_relative(absolute_comp - comp_base);
     }
 
-    ArrayType &operator()(int i, int j, int k, const IndexArray<rank> &ti) const {
+    AMREX_GPU_HOST_DEVICE ArrayType &operator()(int i, int j, int k, const IndexArray<rank> &ti) const {
         return arr(i, j, k, get_absolute(ti));
     }
-    ArrayType &operator()(const amrex::IntVect &iv, const IndexArray<rank> &ti) const {
+    AMREX_GPU_HOST_DEVICE ArrayType &operator()(const amrex::IntVect &iv, const IndexArray<rank> &ti) const {
         return arr(iv, get_absolute(ti));
     }
 
-    Temporary load(int i, int j, int k) const {
+    AMREX_GPU_HOST_DEVICE Temporary load(int i, int j, int k) const {
         Temporary t{};
         for (int c = 0; c < ncomp; ++c) t.data[c] = arr(i, j, k, comp_base + c);
         return t;
     }
-    Temporary load(const amrex::IntVect &iv) const { return load(iv[0], iv[1], iv[2]); }
+    AMREX_GPU_HOST_DEVICE Temporary load(const amrex::IntVect &iv) const { return load(iv[0], iv[1], iv[2]); }
 };
 
 } // namespace treesap::tensor

@@ -38,7 +38,7 @@ template <int n, int ngrow_ = n> struct PhiPiWaveRHS {
     struct Workspace {};
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &) const {
         using namespace treesap::tensor;
         using treesap::stencil::derivative;
@@ -60,7 +60,7 @@ template <int n, int ngrow_ = n> struct PhiUWaveRHS {
     struct Workspace {};
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &) const {
         using namespace treesap::tensor;
         using treesap::stencil::derivative;
@@ -89,7 +89,7 @@ template <int n, int ngrow_ = n> struct NonLinearWaveRHS {
     amrex::Real gamma = 0.0;
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &ws) const {
         using namespace treesap::tensor;
         using treesap::stencil::derivative;
@@ -116,7 +116,7 @@ template <int n, int ngrow_ = n> struct LinearADMRHS {
     struct Workspace {};
 
     template <int comp>
-    amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
+    AMREX_GPU_HOST_DEVICE amrex::Real operator()(const amrex::IntVect idx, amrex::Real, const amrex::Array4<const amrex::Real> &arr,
                            const amrex::GeometryData &geom, Workspace &) const {
         using namespace treesap::tensor;
         using treesap::stencil::derivative;

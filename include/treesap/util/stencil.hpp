@@ -19,7 +19,7 @@ namespace treesap::stencil {
 
 // Reference spelling: the array element type is the second template argument.
 template <int n, typename ArrayType, int... Dirs>
-AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
                                           const amrex::GeometryData &geom) {
     static_assert(((Dirs >= 0 && Dirs <= 2) && ...), "directions must be 0, 1 or 2");
     static_assert(sizeof...(Dirs) >= 1 && sizeof...(Dirs) <= 3, "first, second and third derivatives only");
@@ -28,7 +28,7 @@ AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, i
 
 // Shorter spelling with the element type deduced: derivative<n, Dirs...>(arr, comp, idx, geom).
 template <int n, int... Dirs, typename ArrayType>
-AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
                                           const amrex::GeometryData &geom) {
     static_assert(((Dirs >= 0 && Dirs <= 2) && ...), "directions must be 0, 1 or 2");
     static_assert(sizeof...(Dirs) >= 1 && sizeof...(Dirs) <= 3, "first, second and third derivatives only");
@@ -36,7 +36,7 @@ AMREX_FORCE_INLINE amrex::Real derivative(const amrex::Array4<ArrayType> &arr, i
 }
 
 template <int order, int direction, typename ArrayType>
-AMREX_FORCE_INLINE amrex::Real kreiss_oliger(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real kreiss_oliger(const amrex::Array4<ArrayType> &arr, int comp, const amrex::IntVect &idx,
                                              const amrex::GeometryData &geom) {
     static_assert(order > 0, "Kreiss-Oliger order must be positive");
     using namespace cexprmath;

@@ -78,7 +78,7 @@ template <int n, typename T, int... Dirs> struct derivative_impl {
 };
 
 template <int n, typename T, int a> struct derivative_impl<n, T, a> {
-    static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
+    AMREX_GPU_HOST_DEVICE static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
         const amrex::Real gf = 1.0 / g.CellSize(a);
         amrex::Real acc = 0.0;
         for (int s = 1; s <= n; ++s)
@@ -89,7 +89,7 @@ template <int n, typename T, int a> struct derivative_impl<n, T, a> {
 };
 
 template <int n, typename T, int a, int b> struct derivative_impl<n, T, a, b> {
-    static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
+    AMREX_GPU_HOST_DEVICE static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
         if constexpr (a == b) {
             const amrex::Real gf = 1.0 / (g.CellSize(a) * g.CellSize(b));
             amrex::Real acc = weights::second<n>(0) * u(i, j, k, c);
@@ -110,7 +110,7 @@ template <int n, typename T, int a, int b> struct derivative_impl<n, T, a, b> {
 };
 
 template <int n, typename T, int a, int b, int d> struct derivative_impl<n, T, a, b, d> {
-    static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
+    AMREX_GPU_HOST_DEVICE static amrex::Real eval(int i, int j, int k, int c, const amrex::Array4<T> &u, const amrex::GeometryData &g) {
         if constexpr (a == b && b == d) {
             const amrex::Real gf = 1.0 / (g.CellSize(a) * g.CellSize(a) * g.CellSize(a));
             amrex::Real acc = 0.0;

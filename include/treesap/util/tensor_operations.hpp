@@ -23,7 +23,7 @@ template <typename F, int... Is> auto contract_value(F &f, std::integer_sequence
 // the contraction over the remaining indices with the outer one fixed at I
 template <int I, typename F> struct fix_outer {
     F &f;
-    template <int... Js> auto operator()() const { return f.template operator()<I, Js...>(); }
+    template <int... Js> constexpr auto operator()() const { return f.template operator()<I, Js...>(); }
 };
 } // namespace detail
 

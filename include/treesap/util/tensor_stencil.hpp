@@ -11,7 +11,7 @@ namespace treesap::stencil {
 
 template <int n, int d0, typename AliasType>
     requires(tensor::TensorAliasImplementation<AliasType>)
-AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
                                           const amrex::IntVect &idx, const amrex::GeometryData &geom) {
     using T = typename AliasType::Base::arr_t;
     return stencil_impl::derivative_impl<n, T, d0>::eval(idx[0], idx[1], idx[2], AliasType::Base::get_absolute(ti),
@@ -20,7 +20,7 @@ AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::Inde
 
 template <int n, int d0, int d1, typename AliasType>
     requires(tensor::TensorAliasImplementation<AliasType>)
-AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
                                           const amrex::IntVect &idx, const amrex::GeometryData &geom) {
     using T = typename AliasType::Base::arr_t;
     return stencil_impl::derivative_impl<n, T, d0, d1>::eval(idx[0], idx[1], idx[2], AliasType::Base::get_absolute(ti),
@@ -29,7 +29,7 @@ AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::Inde
 
 template <int n, int d0, int d1, int d2, typename AliasType>
     requires(tensor::TensorAliasImplementation<AliasType>)
-AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real derivative(const AliasType &t, const tensor::IndexArray<AliasType::rank> &ti,
                                           const amrex::IntVect &idx, const amrex::GeometryData &geom) {
     using T = typename AliasType::Base::arr_t;
     return stencil_impl::derivative_impl<n, T, d0, d1, d2>::eval(idx[0], idx[1], idx[2],

@@ -49,7 +49,11 @@ typedef enum tsp_system {
     TSP_SYS_WAVE_PHIPI = 0, /* (phi, Pi): phi_t = Pi, Pi_t = sum_a D_aa phi           */
     TSP_SYS_WAVE_PHIU = 1,  /* (phi, u_a): examples/01-wave-eqn/WaveEqnRHS.hpp:16-50    */
     TSP_SYS_NONLINEAR = 2,  /* (phi, pi): examples/02-temporaries/NonLinearRHS.hpp:16-71 */
-    TSP_SYS_TENSOR_ADM = 3  /* (h_ij, K_ij) linearised ADM, 12 comps, SymRank2 aliases   */
+    TSP_SYS_TENSOR_ADM = 3, /* (h_ij, K_ij) linearised ADM, 12 comps, SymRank2 aliases   */
+    TSP_SYS_CUSTOM = 4      /* any RHSConcept functor: the caller's stage kernel evaluates it
+                               (tsp_rk4_steps_custom); tsp_desc.ncomp components; the KO and
+                               ABL decorators belong to the caller's functor (ko_order and
+                               abl_thickness must be 0) */
 } tsp_system;
 
 /* Physical boundary functor on non-periodic faces (BCConcept,
@@ -81,7 +85,7 @@ typedef struct tsp_desc {
     int32_t periodic[3];    /* Geometry is_periodic */
     int32_t abl_thickness;  /* ABLData::thickness in cells; 0 = no ABL decorator */
     int32_t abl_profile;    /* tsp_profile */
-    int32_t reserved0;
+    int32_t ncomp;          /* component count of a TSP_SYS_CUSTOM model (ignored otherwise) */
     double abl_max;         /* ABLData::max_strength */
     double target[TSP_MAX_COMP]; /* ABLData::target_values == DirichletBC::boundary_values */
     double gamma;           /* NonLinearRHS::gamma (system 2 only) */
@@ -289,6 +293,34 @@ tsp_status tsp_batch_adjoint_forward(tsp_batch_adjoint *a);
 tsp_status tsp_batch_adjoint_trace(tsp_batch_adjoint *a, double *dst, int32_t on_device);
 tsp_status tsp_batch_adjoint_backward(tsp_batch_adjoint *a, const double *dtrace, int32_t dtrace_on_device,
                                       double *grad, int32_t grad_on_device, int32_t recompute);
+
+/* Stepping of a model whose right-hand side the library does not carry
+ * (TSP_SYS_CUSTOM): the library owns the registers, the FillPatch before every
+ * stage (periodic images, then the Dirichlet/NoOp boundary values), the
+ * register roles and the clock, exactly as tsp_rk4_steps / tsp_euler_steps do,
+ * and calls `fn` once per stage to launch the caller's stage kernel on
+ * `hip_stream` (SingleLevelRK4Integrator.hpp:110-157 runStage,
+ * SingleLevelEulerIntegrator.hpp:66-88). The view holds everything that
+ * kernel needs; element (i,j,k,c) of a register, ghosts included
+ * (-G <= i < n + G), is at reg[i + j*sj + k*sk + c*sc]. `fn` returns 0 on
+ * success; any other value stops the stepping with TSP_HIP_ERROR. */
+typedef struct tsp_stage_view {
+    int32_t stage;      /* 0..3: RK4 stage; 4: forward Euler */
+    int32_t n[3];       /* valid cells per axis */
+    int32_t ghost;      /* ghost width G */
+    int32_t ncomp;
+    double time;        /* time + dt * a_fraction[stage] (RK4), time (Euler) */
+    double b;           /* dt * b_fraction[stage] (RK4), dt (Euler) */
+    double a_next;      /* dt * a_fraction[stage + 1] (stages 0..2) */
+    const double *src;  /* register the RHS reads, ghosts filled: init (stage 0, Euler) or arg */
+    const double *init; /* state_old */
+    double *acc;        /* state_new */
+    double *argn;       /* arg_next (stages 0..2), else NULL */
+    int64_t sj, sk, sc; /* element strides, the same for all registers */
+} tsp_stage_view;
+typedef int (*tsp_stage_fn)(void *user, const tsp_stage_view *view, void *hip_stream);
+tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, void *user);
+tsp_status tsp_euler_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, void *user);
 
 /* Algorithmic HBM bytes of one RK4 step per valid cell (128 * ncomp). */
 int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d);

@@ -3,15 +3,19 @@
 dropin_host  host-only checks of the C++ API layer and RHS evaluation for the
              oracle comparison (runs without a GPU);
 dropin_gpu   TreESaP-style programs run through the drop-in integrators on the
-             GPU.
+             GPU, their right-hand sides mapped onto the library's kernels;
+dropin_generic  programs whose right-hand sides the library does not carry,
+             run on the stage kernel compiled from the functor.
 
-Both are ordinary g++ C++20 programs that include include/treesap/ and link
+The first two are ordinary g++ C++20 programs, the last is compiled by hipcc
+for gfx950; all include include/treesap/ and link
 treesap_amd/libtreesap_hip.so (found at run time through an $ORIGIN rpath).
 Usage: python tests/cpp/build.py
 """
 from __future__ import annotations
 
 import concurrent.futures as cf
+import shutil
 import subprocess
 import sys
 from pathlib import Path
@@ -19,13 +23,27 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 BIN = HERE / "bin"
-PROGRAMS = ("dropin_host", "dropin_gpu")
+PROGRAMS = ("dropin_host", "dropin_gpu", "dropin_generic")
+# programs whose right-hand sides run on the stage kernel compiled from the functor
+HIP_PROGRAMS = ("dropin_generic",)
+
+
+def _hipcc() -> str:
+    for cand in ("/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
 
 
 def _compile(name: str) -> Path:
     out = BIN / name
+    if name in HIP_PROGRAMS:
+        head = [_hipcc(), "--offload-arch=gfx950", "-std=c++20", "-O2", "-ffp-contract=off",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
+    else:
+        head = ["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra"]
     cmd = [
-        "g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra",
+        *head,
         f"-I{ROOT / 'include' / 'treesap'}", f"-I{ROOT / 'include'}", f"-I{HERE}",
         str(HERE / f"{name}.cpp"), "-o", str(out),
         f"-L{ROOT / 'treesap_amd'}", "-ltreesap_hip", "-Wl,-rpath,$ORIGIN/../../../treesap_amd",

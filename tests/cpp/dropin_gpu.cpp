@@ -9,7 +9,7 @@
 //
 //   dropin_gpu <outdir> [scenario ...]
 #include "treesap.hpp"
-#include "user_rhs.hpp"
+#include "user_rhs_traits.hpp"
 
 #include <cmath>
 #include <cstdio>

@@ -11,7 +11,7 @@
 // cells are filled like the library's FillPatch (periodic images, then the
 // boundary functor outside non-periodic faces, with Dirichlet value 0).
 #include "treesap.hpp"
-#include "user_rhs.hpp"
+#include "user_rhs_traits.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -102,25 +102,47 @@ static void test_stencil_weights() {
 static void test_recognition() {
     const auto g = cube(16, 0.0, 1.0, 1);
     using device::recognise;
-    auto r1 = recognise(FirstOrderWave<1>{}, g.data());
+    // system_traits mappings of user functors, verified bit for bit over the domain
+    auto r1 = recognise(FirstOrderWave<1>{}, g.data(), 2);
     EXPECT(r1.system == TSP_SYS_WAVE_PHIU && r1.stencil_n == 1);
-    auto r3 = recognise(FirstOrderWave<3>{}, g.data());
+    auto r3 = recognise(FirstOrderWave<3>{}, g.data(), 2);
     EXPECT(r3.system == TSP_SYS_WAVE_PHIU && r3.stencil_n == 3);
-    auto r2 = recognise(ScalarWave<2, 3>{}, g.data());
+    auto r2 = recognise(ScalarWave<2, 3>{}, g.data(), 2);
     EXPECT(r2.system == TSP_SYS_WAVE_PHIPI && r2.stencil_n == 2);
-    auto r4 = recognise(ScalarWave<4, 4>{}, g.data());
+    auto r4 = recognise(ScalarWave<4, 4>{}, g.data(), 2);
     EXPECT(r4.system == TSP_SYS_WAVE_PHIPI && r4.stencil_n == 4);
-    auto rn = recognise(SelfDampedWave<2>{0.3}, g.data());
+    auto rn = recognise(SelfDampedWave<2>{0.3}, g.data(), 2);
     EXPECT(rn.system == TSP_SYS_NONLINEAR && rn.stencil_n == 2 && rn.gamma == 0.3);
-    auto rt = recognise(VacuumADM<3, 3>{}, g.data());
+    auto rt = recognise(VacuumADM<3, 3>{}, g.data(), 2);
     EXPECT(rt.system == TSP_SYS_TENSOR_ADM && rt.stencil_n == 3);
-    auto rt2 = recognise(VacuumADM<1, 2>{}, g.data());
+    auto rt2 = recognise(VacuumADM<1, 2>{}, g.data(), 2);
     EXPECT(rt2.system == TSP_SYS_TENSOR_ADM && rt2.stencil_n == 1);
-    EXPECT(throws([&] { recognise(Advection<1>{}, g.data()); }));
-    EXPECT(throws([&] { recognise(ForcedWave<2>{}, g.data()); }));
+    // wrong mappings are rejected: another operator, a time dependence, and a
+    // position-dependent source that matches the registered system at every
+    // cell of the lower half of the box (the corner region a local check sees)
+    EXPECT(throws([&] { recognise(Advection<1>{}, g.data(), 2); }));
+    EXPECT(throws([&] { recognise(ForcedWave<2>{}, g.data(), 2); }));
+    EXPECT(throws([&] { recognise(PatchedWave<2>{}, g.data(), 2); }));
+    {
+        std::vector<double> f(12 * 12 * 12 * 2);
+        for (std::size_t q = 0; q < f.size(); ++q) f[q] = std::sin(0.37 * (double)q);
+        const amrex::Array4<const double> arr(f.data(), {-2, -2, -2}, {10, 10, 10}, 2);
+        bool same = true;
+        for (int k = 0; k < 8; ++k)
+            for (int j = 0; j < 8; ++j)
+                for (int i = 0; i < 8; ++i) {
+                    PatchedWave<2>::Workspace w1{};
+                    systems::PhiPiWaveRHS<2, 2>::Workspace w2{};
+                    const double a = PatchedWave<2>{}.operator()<1>({i, j, k}, 0.0, arr, g.data(), w1);
+                    const double b = systems::PhiPiWaveRHS<2, 2>{}.operator()<1>({i, j, k}, 0.0, arr, g.data(), w2);
+                    same = same && a == b;
+                }
+        EXPECT(same);
+    }
     // library functors map without probing
-    auto rl = recognise(systems::NonLinearWaveRHS<3, 4>{0.7}, g.data());
+    auto rl = recognise(systems::NonLinearWaveRHS<3, 4>{0.7}, g.data(), 2);
     EXPECT(rl.system == TSP_SYS_NONLINEAR && rl.stencil_n == 3 && rl.gamma == 0.7);
+    static_assert(!device::is_mapped<CurlWave<3>> && device::is_mapped<ScalarWave<2, 2>>);
 }
 
 static void test_descriptors() {
@@ -157,6 +179,17 @@ static void test_descriptors() {
         PDEModel<RHS, ConstInit<12>, BC> m({}, cube(24, 0.0, 1.0, 0), 0.001, RHS{}, ConstInit<12>{0.0}, bc);
         const tsp_desc d = device::make_desc(m);
         EXPECT(d.system == TSP_SYS_TENSOR_ADM && d.bc == TSP_BC_DIRICHLET && d.target[11] == 5.5);
+    }
+    // a functor the library carries no kernel for maps onto the generic kernel, decorators included
+    {
+        using RHS = KODecorator<CurlWave<3>, 2>;
+        using BC = DirichletBC<4, 3>;
+        PDEModel<RHS, ConstInit<4>, BC> m({}, cube(20, 0.0, 1.0, 0), 0.002, RHS{{}, 0.1}, ConstInit<4>{0.0},
+                                          BC{0.5, 0.25, 0.0, -1.0});
+        const tsp_desc d = device::make_desc(m);
+        EXPECT(d.system == TSP_SYS_CUSTOM && d.ncomp == 4 && d.ghost == 3 && d.ko_order == 0);
+        EXPECT(d.abl_thickness == 0 && d.bc == TSP_BC_DIRICHLET && d.target[0] == 0.5 && d.target[3] == -1.0);
+        EXPECT(d.dt == 0.002 && d.n[1] == 20);
     }
     // unsupported profile and mismatched targets are rejected
     {

@@ -133,3 +133,30 @@ def test_batch_validation():
     assert L.tsp_batch_create(None, 0, None, None) == _lib.INVALID_ARG
     assert L.tsp_batch_rk4_steps(None, 1) == _lib.INVALID_ARG
     assert L.tsp_batch_destroy(None) == _lib.OK
+
+
+def test_custom_model_validation():
+    """A TSP_SYS_CUSTOM descriptor (generic kernel) states its component count and carries no
+    KO or ABL decorator of the library's (they belong to the caller's functor); the library's
+    stage kernels refuse it."""
+    L = _lib.lib()
+
+    def status(**kw):
+        d = _lib.TspDesc()
+        assert L.tsp_desc_default(C.byref(d), 4, 16) == _lib.OK
+        assert d.system == 4 and d.ncomp == 1
+        for k, v in kw.items():
+            setattr(d, k, v)
+        h = C.c_void_p()
+        st = L.tsp_create(C.byref(d), None, C.byref(h))
+        return st, L.tsp_last_error().decode()
+
+    st, msg = status(ncomp=0)
+    assert st == _lib.INVALID_ARG and "ncomp in 1..12" in msg
+    st, msg = status(ncomp=13)
+    assert st == _lib.INVALID_ARG and "ncomp in 1..12" in msg
+    st, msg = status(ncomp=3, ko_order=2)
+    assert st == _lib.INVALID_ARG and "decorators are part of its functor" in msg
+    st, msg = status(ncomp=3, abl_thickness=2)
+    assert st == _lib.INVALID_ARG and "decorators are part of its functor" in msg
+    assert L.tsp_rk4_steps_custom(None, 1, None, None) == _lib.INVALID_ARG

@@ -250,3 +250,57 @@ def test_dropin_plotfiles(gpu_runs):
     assert np.array_equal(np.frombuffer(data, dtype=np.float64).reshape(case.shape), o.get_state())
     hdr = (d / "plt00004" / "Header").read_text().splitlines()
     assert hdr[0] == "HyperCLaw-V1.1" and hdr[1] == "4" and hdr[2:6] == ["phi", "u_0", "u_1", "u_2"]
+
+
+# ---- functors the library does not carry: the generic kernel (tests/cpp/dropin_generic.cpp) ----
+
+GENERIC_SCENARIOS = ["first_order", "abl_tensor", "nonlinear_euler", "curl_rk4", "curl_euler", "patched"]
+_Z = (0.0,) * 12
+# scenarios whose functors restate a system the oracle carries (same arithmetic)
+GENERIC_ORACLE = {
+    "first_order": Case(system="phiu", n=(24, 20, 22), stencil_n=2, ghost=2, ko_order=2, ko_strength=0.1,
+                        periodic=(0, 0, 0), bc="dirichlet", target=(0.1, -0.2, 0.3, 0.05) + _Z[4:], dt=0.25 / 24),
+    "abl_tensor": Case(system="tensor", n=(20, 18, 16), prob_hi=(1.0, 0.9, 0.8), stencil_n=2, ghost=2, ko_order=2,
+                       ko_strength=0.1, periodic=(0, 0, 0), bc="dirichlet", abl_thickness=3, abl_max=1.5,
+                       abl_profile="quadratic", dt=0.2 / 20),
+    "nonlinear_euler": Case(system="nonlinear", n=(26, 22, 18), prob_hi=(1.0, 0.9, 0.8), stencil_n=2, ghost=2,
+                            ko_order=2, ko_strength=0.1, periodic=(1, 1, 1), bc="noop", gamma=0.05, dt=0.004),
+}
+
+
+@pytest.fixture(scope="module")
+def generic_runs(tmp_path_factory):
+    out = tmp_path_factory.mktemp("generic")
+    print(f"[dropin_generic] scenarios {GENERIC_SCENARIOS} -> {out}", flush=True)
+    _run([cpp_build.binary("dropin_generic"), out, *GENERIC_SCENARIOS])
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", GENERIC_SCENARIOS)
+def test_generic_kernel_program(generic_runs, scenario):
+    """Unmapped right-hand sides run on the stage kernel compiled from the functor
+    (descriptor system TSP_SYS_CUSTOM). Restatements of the oracle's systems match
+    the oracle bitwise; functors with third derivatives, a curl, antisymmetric and
+    cyclic parts, position and time dependence, or a source confined to half the
+    box match the same functor stepped on the host (tests/cpp/host_rk4.hpp) bitwise."""
+    d = generic_runs / scenario
+    kind, steps, time, system = (d / "meta.txt").read_text().split()
+    assert int(system) == 4, "the generic kernel did not run"
+    steps = int(steps)
+    final = np.fromfile(d / "final.bin", dtype=np.float64)
+    init = np.fromfile(d / "init.bin", dtype=np.float64)
+    if scenario in GENERIC_ORACLE:
+        case = GENERIC_ORACLE[scenario]
+        o = orc.OracleSolver(case)
+        o.set_state(init.reshape(case.shape))
+        (o.rk4_steps if kind == "rk4" else o.euler_steps)(steps)
+        want = o.get_state().ravel()
+        t = 0.0
+        for _ in range(steps):
+            t += case.dt
+        assert float(time) == t
+    else:
+        want = np.fromfile(d / "host.bin", dtype=np.float64)
+    assert np.isfinite(final).all() and not np.array_equal(final, init)
+    assert np.array_equal(final, want), f"{scenario}: max abs diff {np.max(np.abs(final - want))}"

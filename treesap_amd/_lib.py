@@ -44,7 +44,7 @@ class TspDesc(C.Structure):
         ("periodic", C.c_int32 * 3),
         ("abl_thickness", C.c_int32),
         ("abl_profile", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("ncomp", C.c_int32),
         ("abl_max", C.c_double),
         ("target", C.c_double * TSP_MAX_COMP),
         ("gamma", C.c_double),
@@ -135,6 +135,9 @@ PROTOTYPES = {
     "tsp_adjoint_checkpoint": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32]),
     "tsp_adjoint_backward": (C.c_int, [_VP, _VP, C.c_int32, _VP, C.c_int32, C.c_int32]),
     "tsp_adjoint_apply": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
+    # stage callback: int (*)(void *user, const tsp_stage_view *view, void *hip_stream)
+    "tsp_rk4_steps_custom": (C.c_int, [_VP, C.c_int64, _VP, _VP]),
+    "tsp_euler_steps_custom": (C.c_int, [_VP, C.c_int64, _VP, _VP]),
     "tsp_batch_adjoint_create": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(_VP)]),
     "tsp_batch_adjoint_destroy": (C.c_int, [_VP]),

@@ -66,7 +66,8 @@ namespace {
 // least two workgroups per CU, so small boxes fill the device too.
 tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1, 0};
-    if (d.system == TSP_SYS_TENSOR_ADM) c.ghost_mode = 1; // mixed stencils read filled edge/corner ghosts
+    // mixed stencils read filled edge/corner ghosts; a custom kernel reads ghost memory
+    if (d.system == TSP_SYS_TENSOR_ADM || d.system == TSP_SYS_CUSTOM) c.ghost_mode = 1;
     const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
     for (int zc : {64, 32, 16, 8}) {
         c.zchunk = std::min(zc, d.n[2]);
@@ -77,7 +78,9 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
 
 // The tensor system's stage kernel reads ghost memory (edges and corners for
 // its mixed stencils), so it always runs with the fill pass.
-bool fills_ghosts(const tsp_solver *s) { return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM; }
+bool fills_ghosts(const tsp_solver *s) {
+    return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM || s->d.system == TSP_SYS_CUSTOM;
+}
 
 double *reg_origin(const tsp_solver *s, int reg) {
     return s->base[reg] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
@@ -191,6 +194,7 @@ struct StepTiming {
 // Enqueue one RK4 step on the solver's stream (SingleLevelIntegrator.hpp:102-125,
 // SingleLevelRK4Integrator.hpp:95-157).
 tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
+    if (!s->launch) return fail(TSP_UNSUPPORTED, "a custom model steps through tsp_rk4_steps_custom");
     const double dt = s->d.dt;
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
         &ARGN = s->role[TSP_ROLE_ARG_NEXT];
@@ -258,6 +262,7 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
 }
 
 tsp_status enqueue_euler_step(tsp_solver *s) {
+    if (!s->launch) return fail(TSP_UNSUPPORTED, "a custom model steps through tsp_euler_steps_custom");
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
     if (fills_ghosts(s)) TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
     const tsp::StageArgs a = stage_args(s, tsp::kEuler, OLD, OLD, NEW, -1, s->d.dt, 0.0);
@@ -326,15 +331,16 @@ const char *tsp_last_error(void) { return g_err.c_str(); }
 const char *tsp_version(void) { return TSP_VERSION; }
 
 int64_t tsp_rk4_bytes_per_cell(const tsp_desc *d) {
-    const int nc = d ? system_ncomp(d->system) : -1;
+    const int nc = d ? desc_ncomp(*d) : -1;
     return nc < 0 ? -1 : 128LL * nc;
 }
 
 tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n) {
     if (!d) return fail(TSP_INVALID_ARG, "descriptor is NULL");
-    if (system_ncomp(system) < 0) return fail(TSP_INVALID_ARG, "unknown system id");
+    if (system != TSP_SYS_CUSTOM && system_ncomp(system) < 0) return fail(TSP_INVALID_ARG, "unknown system id");
     std::memset(d, 0, sizeof(*d));
     d->system = system;
+    if (system == TSP_SYS_CUSTOM) d->ncomp = 1;
     for (int a = 0; a < 3; ++a) {
         d->n[a] = n;
         d->prob_lo[a] = 0.0;
@@ -359,7 +365,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     tsp_solver *s = new (std::nothrow) tsp_solver();
     if (!s) return fail(TSP_OOM, "host allocation failed");
     s->d = *d;
-    s->nc = system_ncomp(d->system);
+    s->nc = desc_ncomp(*d);
     s->G = d->ghost;
     s->H = std::max(d->stencil_n, d->ko_order);
     s->pitch = (int)(((int64_t)kXOff + d->n[0] + s->G + 15) / 16 * 16);
@@ -524,6 +530,77 @@ tsp_status tsp_euler_steps(tsp_solver *s, int64_t nsteps) {
     for (int64_t n = 0; n < nsteps; ++n) {
         tsp_status st = enqueue_euler_step(s);
         if (st != TSP_OK) return st;
+        advance_clock(s);
+    }
+    return TSP_OK;
+}
+
+namespace {
+tsp_stage_view stage_view(const tsp_solver *s, int stage, int src, int init, int acc, int argn, double time, double b,
+                          double a_next) {
+    tsp_stage_view v{};
+    v.stage = stage;
+    for (int a = 0; a < 3; ++a) v.n[a] = s->d.n[a];
+    v.ghost = s->G;
+    v.ncomp = s->nc;
+    v.time = time;
+    v.b = b;
+    v.a_next = a_next;
+    v.src = reg_origin(s, src);
+    v.init = reg_origin(s, init);
+    v.acc = reg_origin(s, acc);
+    v.argn = argn >= 0 ? reg_origin(s, argn) : nullptr;
+    v.sj = s->sj;
+    v.sk = s->sk;
+    v.sc = s->sc;
+    return v;
+}
+
+tsp_status call_stage(tsp_solver *s, tsp_stage_fn fn, void *user, const tsp_stage_view &v) {
+    const int r = fn(user, &v, (void *)s->stream);
+    if (r != 0) return fail(TSP_HIP_ERROR, "custom stage kernel failed (callback returned " + std::to_string(r) + ")");
+    return TSP_OK;
+}
+} // namespace
+
+tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, void *user) {
+    if (!s || !fn) return fail(TSP_INVALID_ARG, "solver or stage callback is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    const double dt = s->d.dt;
+    int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
+        &ARGN = s->role[TSP_ROLE_ARG_NEXT];
+    for (int64_t n = 0; n < nsteps; ++n) {
+        TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream)); // pre_step FillPatch
+        for (int stage = 0; stage < 4; ++stage) {
+            // integrators/SingleLevelRK4Integrator.hpp:115-125
+            const double a_next = dt * kAFrac[(stage + 1) % 4];
+            const double time_arg = s->time + dt * kAFrac[stage];
+            const double b = dt * kBFrac[stage];
+            if (stage > 0) {
+                std::swap(ARG, ARGN);
+                TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, ARG), s->stream));
+            }
+            const tsp_stage_view v =
+                stage_view(s, stage, stage == 0 ? OLD : ARG, OLD, NEW, stage < 3 ? ARGN : -1, time_arg, b, a_next);
+            tsp_status st = call_stage(s, fn, user, v);
+            if (st != TSP_OK) return st;
+        }
+        std::swap(OLD, NEW); // post_step
+        advance_clock(s);
+    }
+    return TSP_OK;
+}
+
+tsp_status tsp_euler_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, void *user) {
+    if (!s || !fn) return fail(TSP_INVALID_ARG, "solver or stage callback is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
+    for (int64_t n = 0; n < nsteps; ++n) {
+        TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
+        const tsp_stage_view v = stage_view(s, tsp::kEuler, OLD, OLD, NEW, -1, s->time, s->d.dt, 0.0);
+        tsp_status st = call_stage(s, fn, user, v);
+        if (st != TSP_OK) return st;
+        std::swap(OLD, NEW);
         advance_clock(s);
     }
     return TSP_OK;
@@ -732,6 +809,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     if (steps < 1) steps = 1;
     const std::string key = tune_key(s);
     tsp_kernel_cfg best = s->cfg;
+    if (s->d.system == TSP_SYS_CUSTOM) return fail(TSP_UNSUPPORTED, "a custom model has no library stage kernel to tune");
     if (s->d.system == TSP_SYS_TENSOR_ADM) { // one launch shape; nothing to tune
         if (chosen) *chosen = best;
         return TSP_OK;

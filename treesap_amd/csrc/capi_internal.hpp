@@ -46,6 +46,12 @@ inline int system_ncomp(int system) {
     }
 }
 
+// component count of a model (a custom model states its own)
+inline int desc_ncomp(const tsp_desc &d) {
+    if (d.system == TSP_SYS_CUSTOM) return d.ncomp >= 1 && d.ncomp <= TSP_MAX_COMP ? d.ncomp : -1;
+    return system_ncomp(d.system);
+}
+
 inline tsp::LaunchFn lookup_launcher(int system, int n, int r) {
     switch (system) {
     case TSP_SYS_WAVE_PHIPI: return tsp::lookup_phipi(n, r);
@@ -123,22 +129,28 @@ inline Layout layout_of(const tsp_desc &d) {
     L.sj = L.pitch;
     L.sk = L.sj * (d.n[1] + 2 * L.G);
     L.sc = L.sk * (d.n[2] + 2 * L.G);
-    L.elems = L.sc * system_ncomp(d.system);
+    L.elems = L.sc * desc_ncomp(d);
     return L;
 }
 
 inline tsp_status validate(const tsp_desc *d) {
     if (!d) return fail(TSP_INVALID_ARG, "descriptor is NULL");
-    const int nc = system_ncomp(d->system);
+    if (d->system == TSP_SYS_CUSTOM && desc_ncomp(*d) < 0)
+        return fail(TSP_INVALID_ARG, "a custom model needs ncomp in 1..12");
+    const int nc = desc_ncomp(*d);
     if (nc < 0) return fail(TSP_INVALID_ARG, "unknown system id " + std::to_string(d->system));
     for (int a = 0; a < 3; ++a)
         if (d->n[a] < 1) return fail(TSP_INVALID_ARG, "n[" + std::to_string(a) + "] must be >= 1");
-    if (d->stencil_n < 1 || d->stencil_n > 4) return fail(TSP_INVALID_ARG, "stencil_n must be in 1..4");
+    const bool custom = d->system == TSP_SYS_CUSTOM;
+    if (custom && (d->ko_order != 0 || d->abl_thickness != 0))
+        return fail(TSP_INVALID_ARG, "a custom model's KO and ABL decorators are part of its functor "
+                                     "(ko_order and abl_thickness must be 0)");
+    if (!custom && (d->stencil_n < 1 || d->stencil_n > 4)) return fail(TSP_INVALID_ARG, "stencil_n must be in 1..4");
     if (d->ko_order < 0 || d->ko_order > 4) return fail(TSP_INVALID_ARG, "ko_order must be in 0..4");
-    if (d->ghost < d->stencil_n || d->ghost < d->ko_order)
+    if (d->ghost < 0 || (!custom && (d->ghost < d->stencil_n || d->ghost < d->ko_order)))
         return fail(TSP_INVALID_ARG, "ghost width must be >= stencil_n and >= ko_order (KODecorator.hpp:43)");
     if (d->ghost > kXOff) return fail(TSP_INVALID_ARG, "ghost width must be <= 16");
-    if (!lookup_launcher(d->system, d->stencil_n, d->ko_order))
+    if (!custom && !lookup_launcher(d->system, d->stencil_n, d->ko_order))
         return fail(TSP_UNSUPPORTED, "no stage kernel for stencil_n=" + std::to_string(d->stencil_n) +
                                          " ko_order=" + std::to_string(d->ko_order) +
                                          (d->system == TSP_SYS_TENSOR_ADM ? " (supported: ko_order in {0, n})"
