@@ -236,6 +236,29 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
 /* Capture and instantiate, without launching, the two-step graphs of both step
  * parities for the current configuration. */
 tsp_status tsp_batch_graph_prepare(tsp_batch *b);
+/* `nsteps` forward Euler steps of every instance (SingleLevelEulerIntegrator.hpp:66-88). */
+tsp_status tsp_batch_euler_steps(tsp_batch *b, int64_t nsteps);
+
+/* A level split into boxes: replaces the BoxArray that
+ * SingleLevelIntegrator(model, block_size) builds (SingleLevelIntegrator.hpp:47-49,
+ * 172-176: the domain chopped into boxes of at most block_size cells per axis)
+ * with its FillPatch across boxes (SingleLevelIntegrator.hpp:153-163). Each axis
+ * is cut into the fewest pieces of at most block[a] cells, as equal as possible;
+ * the boxes are the instances of a tsp_batch (box b = bx + nbx*(by + nby*bz)),
+ * so tsp_batch_rk4_steps / _graph_rk4_steps / _euler_steps / _set_kernel /
+ * _time_rk4 / _get_time apply. Before every stage the source register's
+ * ghosts are exchanged between the boxes (neighbours, periodic images, edges
+ * and corners) and the Dirichlet values are written outside non-periodic
+ * faces; the stage kernels then read ghost memory. The absorbing layer is
+ * measured in domain coordinates. Results are bit-identical to a single box
+ * of the whole domain for every block size. (phi, Pi) system only. */
+tsp_status tsp_level_create(const tsp_desc *d, const int32_t block[3], void *stream, tsp_batch **out);
+/* Box b's index range, lo_hi[6*b + 0..2] = lo (x, y, z), [6*b + 3..5] = hi (inclusive). */
+tsp_status tsp_level_boxes(tsp_batch *b, int32_t *lo_hi);
+/* The whole domain's valid state, layout as tsp_set_state; setting it restarts
+ * the clock (initialise()). */
+tsp_status tsp_level_set_state(tsp_batch *b, const double *src, int32_t on_device);
+tsp_status tsp_level_get_state(tsp_batch *b, double *dst, int32_t on_device);
 tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms);
 tsp_status tsp_batch_sync(tsp_batch *b);
 

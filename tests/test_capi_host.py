@@ -160,3 +160,15 @@ def test_custom_model_validation():
     st, msg = status(ncomp=3, abl_thickness=2)
     assert st == _lib.INVALID_ARG and "decorators are part of its functor" in msg
     assert L.tsp_rk4_steps_custom(None, 1, None, None) == _lib.INVALID_ARG
+
+
+def test_level_validation():
+    """tsp_level_create rejects bad block sizes and systems other than (phi, Pi) before any HIP call."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    d = make_desc(Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01))
+    assert L.tsp_level_create(C.byref(d), (C.c_int32 * 3)(8, 0, 8), None, C.byref(h)) == _lib.INVALID_ARG
+    assert L.tsp_level_create(C.byref(d), None, None, C.byref(h)) == _lib.INVALID_ARG
+    d2 = make_desc(Case(system="phiu", n=(16, 16, 16), stencil_n=2, ghost=2, dt=0.01))
+    assert L.tsp_level_create(C.byref(d2), (C.c_int32 * 3)(8, 8, 8), None, C.byref(h)) == _lib.UNSUPPORTED
+    assert "(phi,Pi)" in L.tsp_last_error().decode()

@@ -5,8 +5,8 @@ include/treesap_hip.h); this package holds its ctypes binding, the host-side
 solver object and the seeded workload definitions.
 """
 from . import _lib
-from .solver import Batch, Solver
+from .solver import Batch, Level, Solver
 from .workloads import Case, make_desc
 
-__all__ = ["Batch", "Solver", "Case", "make_desc", "_lib"]
+__all__ = ["Batch", "Level", "Solver", "Case", "make_desc", "_lib"]
 __version__ = "0.1.0"

@@ -51,9 +51,20 @@ struct tsp_batch {
     tsp_batch_cfg work_cfg{-1, -1, -1, -1, -1, -1}; // configuration the work list was built for
     std::vector<double> time;
     int64_t steps = 0;
+    // two-step graphs, each for the register roles it was captured with
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_role[2][4] = {};
+    int graph_next = 0;
     tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1, -1};
     uint64_t work_gen = 0; // bumped whenever the work list is rebuilt (graphs of other objects hold it)
+    // a level (tsp_level_create): the instances are the boxes of one domain and
+    // every stage starts with the ghost exchange between them
+    bool level = false;
+    tsp_desc ldesc{};            // the domain
+    tsp::LevelArgs largs{};      // device view of the box layout
+    int32_t *d_axis_box = nullptr;
+    long long max_shell = 0;     // ghost cells of the largest box
+    std::vector<int32_t> box_lo; // 3 per box
 };
 
 namespace {
@@ -76,6 +87,15 @@ tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
 
 int auto_zchunk(int nz) { return std::min(nz, 32); }
 
+constexpr int kIdentityRoles[4] = {0, 1, 2, 3};
+// register roles one RK4 step after `r` (old <-> new, arg <-> arg_next)
+void rk4_next_roles(const int r[4], int out[4]) {
+    out[TSP_ROLE_OLD] = r[TSP_ROLE_NEW];
+    out[TSP_ROLE_NEW] = r[TSP_ROLE_OLD];
+    out[TSP_ROLE_ARG] = r[TSP_ROLE_ARG_NEXT];
+    out[TSP_ROLE_ARG_NEXT] = r[TSP_ROLE_ARG];
+}
+
 void drop_batch_graphs(tsp_batch *b) {
     for (auto &g : b->graph)
         if (g) {
@@ -83,6 +103,7 @@ void drop_batch_graphs(tsp_batch *b) {
             g = nullptr;
         }
     b->graph_cfg = tsp_batch_cfg{-1, -1, -1, -1, -1, -1};
+    b->graph_next = 0;
 }
 
 // (Re)build the device work list: instances in decreasing cell count, each
@@ -142,6 +163,8 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
         &ARGN = b->role[TSP_ROLE_ARG_NEXT];
     for (int stage = 0; stage < 4; ++stage) {
         if (stage > 0) std::swap(ARG, ARGN);
+        if (b->level) // FillPatch of the stage's source register across the boxes
+            TSP_HIP(tsp::launch_level_exchange(b->largs, stage == 0 ? OLD : ARG, b->max_shell, b->stream));
         for (int v = 0; v < tsp::kBatchVariants; ++v) {
             if (b->group_n[v] == 0) continue;
             tsp::BatchArgs a{};
@@ -154,10 +177,34 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
             a.acc = NEW;
             a.argn = stage < 3 ? ARGN : -1;
             a.zalt = b->cfg.zalt;
-            a.ghost_mode = 0;
+            a.ghost_mode = b->level ? 1 : 0; // a box's ghosts hold its neighbours' values after the exchange
             const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
             TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
         }
+    }
+    std::swap(OLD, NEW);
+    return TSP_OK;
+}
+
+// One forward Euler step of every instance (SingleLevelEulerIntegrator.hpp:66-88).
+tsp_status enqueue_batch_euler_step(tsp_batch *b) {
+    int &OLD = b->role[TSP_ROLE_OLD], &NEW = b->role[TSP_ROLE_NEW];
+    if (b->level) TSP_HIP(tsp::launch_level_exchange(b->largs, OLD, b->max_shell, b->stream));
+    for (int v = 0; v < tsp::kBatchVariants; ++v) {
+        if (b->group_n[v] == 0) continue;
+        tsp::BatchArgs a{};
+        a.inst = b->d_inst;
+        a.work = b->d_work + b->group_off[v];
+        a.nwork = b->group_n[v];
+        a.stage = tsp::kEuler;
+        a.src = OLD;
+        a.init = OLD;
+        a.acc = NEW;
+        a.argn = -1;
+        a.zalt = b->cfg.zalt;
+        a.ghost_mode = b->level ? 1 : 0;
+        const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
+        TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
     }
     std::swap(OLD, NEW);
     return TSP_OK;
@@ -168,23 +215,29 @@ void batch_advance_clock(tsp_batch *b) {
     for (int i = 0; i < b->count; ++i) b->time[i] += b->d[i].dt;
 }
 
-// register roles after an even / odd number of RK4 steps (one step swaps
-// old <-> new and arg <-> arg_next)
-constexpr int kEvenRoles[4] = {0, 1, 2, 3};
-constexpr int kOddRoles[4] = {1, 0, 3, 2};
 
-// The captured two-step graph for step parity `parity` (captured with that
-// parity's register roles if missing); graphs of another configuration are
-// dropped first.
-tsp_status ensure_batch_graph(tsp_batch *b, int parity) {
+// The captured two-step graph for register roles `roles` (captured with those
+// roles if missing; two RK4 steps bring the roles back); graphs of another
+// configuration are dropped first.
+tsp_status ensure_batch_graph(tsp_batch *b, const int roles[4], int *slot) {
     if (std::memcmp(&b->graph_cfg, &b->cfg, sizeof(tsp_batch_cfg)) != 0) {
         drop_batch_graphs(b);
         b->graph_cfg = b->cfg;
     }
-    if (b->graph[parity]) return TSP_OK;
+    for (int g = 0; g < 2; ++g)
+        if (b->graph[g] && std::memcmp(b->graph_role[g], roles, sizeof(b->graph_role[g])) == 0) {
+            *slot = g;
+            return TSP_OK;
+        }
+    const int g = !b->graph[0] ? 0 : !b->graph[1] ? 1 : b->graph_next;
+    b->graph_next = 1 - g;
+    if (b->graph[g]) {
+        (void)hipGraphExecDestroy(b->graph[g]);
+        b->graph[g] = nullptr;
+    }
     int saved[4];
     std::memcpy(saved, b->role, sizeof(saved));
-    std::memcpy(b->role, parity ? kOddRoles : kEvenRoles, sizeof(saved));
+    std::memcpy(b->role, roles, sizeof(saved));
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
         std::memcpy(b->role, saved, sizeof(saved));
@@ -200,25 +253,31 @@ tsp_status ensure_batch_graph(tsp_batch *b, int parity) {
         return st;
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&b->graph[parity], graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&b->graph[g], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        b->graph[parity] = nullptr;
+        b->graph[g] = nullptr;
         return hip_fail(e, "hipGraphInstantiate");
     }
+    std::memcpy(b->graph_role[g], roles, sizeof(b->graph_role[g]));
+    *slot = g;
     return TSP_OK;
 }
-
 } // namespace
 
 extern "C" {
 
-tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out) {
+namespace {
+// A batch of `count` instances. `box_off`/`dom` (3 per instance / 3): the
+// instances are the boxes of one validated domain (a level): per-instance
+// descriptors are then only checked for what the batched kernels need.
+tsp_status create_batch(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out, const int32_t *box_off,
+                        const int32_t *dom) {
     if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
     *out = nullptr;
     if (!descs || count < 1) return fail(TSP_INVALID_ARG, "need at least one instance");
     for (int i = 0; i < count; ++i) {
-        tsp_status st = validate(&descs[i]);
+        tsp_status st = box_off ? TSP_OK : validate(&descs[i]);
         if (st != TSP_OK) return fail(st, "instance " + std::to_string(i) + ": " + g_err);
         if (descs[i].system != TSP_SYS_WAVE_PHIPI)
             return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": batched mode runs the (phi,Pi) system");
@@ -294,10 +353,15 @@ tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, 
         for (int a = 0; a < 3; ++a) I.periodic[a] = d.periodic[a];
         I.dirichlet = d.bc == TSP_BC_DIRICHLET;
         I.abl_T = d.abl_thickness;
+        for (int a = 0; a < 3; ++a) {
+            I.off[a] = box_off ? box_off[3 * i + a] : 0;
+            I.dom[a] = dom ? dom[a] : d.n[a];
+        }
         b->max_abl_T = std::max(b->max_abl_T, d.abl_thickness);
         I.dt = d.dt;
         for (int a = 0; a < 3; ++a) {
-            const double dx = (d.prob_hi[a] - d.prob_lo[a]) / (double)d.n[a];
+            // Geometry::CellSize of the domain (a box of a level keeps the domain's prob_lo/hi)
+            const double dx = (d.prob_hi[a] - d.prob_lo[a]) / (double)(dom ? dom[a] : d.n[a]);
             I.gf1[a] = 1.0 / dx;
             I.gf2[a] = 1.0 / (dx * dx);
         }
@@ -332,6 +396,11 @@ tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, 
     *out = b;
     return TSP_OK;
 }
+} // namespace
+
+tsp_status tsp_batch_create(const tsp_desc *descs, int32_t count, void *stream, tsp_batch **out) {
+    return create_batch(descs, count, stream, out, nullptr, nullptr);
+}
 
 tsp_status tsp_batch_destroy(tsp_batch *b) {
     if (!b) return TSP_OK;
@@ -342,6 +411,7 @@ tsp_status tsp_batch_destroy(tsp_batch *b) {
         if (p) (void)hipFree(p);
     if (b->d_inst) (void)hipFree(b->d_inst);
     if (b->d_work) (void)hipFree(b->d_work);
+    if (b->d_axis_box) (void)hipFree(b->d_axis_box);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TSP_OK;
@@ -439,16 +509,27 @@ tsp_status tsp_batch_rk4_steps(tsp_batch *b, int64_t nsteps) {
     return st;
 }
 
+tsp_status tsp_batch_euler_steps(tsp_batch *b, int64_t nsteps) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    tsp_status st = build_work(b);
+    for (int64_t n = 0; n < nsteps && st == TSP_OK; ++n) {
+        st = enqueue_batch_euler_step(b);
+        if (st == TSP_OK) batch_advance_clock(b);
+    }
+    return st;
+}
+
 tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps) {
     if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
     tsp_status st = build_work(b);
     if (st != TSP_OK) return st;
-    const int parity = (int)(b->steps & 1);
-    if (nsteps >= 2 && (st = ensure_batch_graph(b, parity)) != TSP_OK) return st;
+    int slot = -1;
+    if (nsteps >= 2 && (st = ensure_batch_graph(b, b->role, &slot)) != TSP_OK) return st;
     int64_t n = 0;
     for (; n + 2 <= nsteps; n += 2) {
-        TSP_HIP(hipGraphLaunch(b->graph[parity], b->stream));
+        TSP_HIP(hipGraphLaunch(b->graph[slot], b->stream));
         batch_advance_clock(b);
         batch_advance_clock(b);
     }
@@ -459,7 +540,10 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps) {
 tsp_status tsp_batch_graph_prepare(tsp_batch *b) {
     if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
     tsp_status st = build_work(b);
-    for (int p = 0; p < 2 && st == TSP_OK; ++p) st = ensure_batch_graph(b, p);
+    int slot = -1, next[4];
+    rk4_next_roles(b->role, next);
+    if (st == TSP_OK) st = ensure_batch_graph(b, b->role, &slot);
+    if (st == TSP_OK) st = ensure_batch_graph(b, next, &slot);
     return st;
 }
 
@@ -491,6 +575,135 @@ tsp_status tsp_batch_sync(tsp_batch *b) {
     return TSP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// a level split into boxes (SingleLevelIntegrator.hpp:47-49,172-176)
+// ---------------------------------------------------------------------------
+
+// Cut [0, len) into pieces of at most `bs` cells: the fewest pieces, sizes as
+// equal as possible (the larger ones first). Returns the piece starts and len.
+static std::vector<int> chop_axis(int len, int bs) {
+    const int nb = (len + bs - 1) / bs;
+    const int sz = len / nb, extra = len % nb;
+    std::vector<int> cuts{0};
+    for (int p = 0; p < nb; ++p) cuts.push_back(cuts.back() + sz + (p < extra ? 1 : 0));
+    return cuts;
+}
+
+tsp_status tsp_level_create(const tsp_desc *d, const int32_t *block, void *stream, tsp_batch **out) {
+    if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!block) return fail(TSP_INVALID_ARG, "block size is NULL");
+    tsp_status st = validate(d);
+    if (st != TSP_OK) return st;
+    if (d->system != TSP_SYS_WAVE_PHIPI)
+        return fail(TSP_UNSUPPORTED, "multi-box levels run the (phi,Pi) system");
+    for (int a = 0; a < 3; ++a)
+        if (block[a] < 1) return fail(TSP_INVALID_ARG, "block size entries must be >= 1");
+    std::vector<int> cuts[3];
+    for (int a = 0; a < 3; ++a) cuts[a] = chop_axis(d->n[a], block[a]);
+    const int nb[3] = {(int)cuts[0].size() - 1, (int)cuts[1].size() - 1, (int)cuts[2].size() - 1};
+    const long long nboxes = (long long)nb[0] * nb[1] * nb[2];
+    if (nboxes > 65535) return fail(TSP_UNSUPPORTED, "a level holds at most 65535 boxes");
+    std::vector<tsp_desc> descs;
+    std::vector<int32_t> offs;
+    for (int bz = 0; bz < nb[2]; ++bz)
+        for (int by = 0; by < nb[1]; ++by)
+            for (int bx = 0; bx < nb[0]; ++bx) { // instance b = bx + nb0 * (by + nb1 * bz)
+                tsp_desc bd = *d;
+                const int lo[3] = {cuts[0][bx], cuts[1][by], cuts[2][bz]};
+                const int hi[3] = {cuts[0][bx + 1], cuts[1][by + 1], cuts[2][bz + 1]};
+                for (int a = 0; a < 3; ++a) {
+                    bd.n[a] = hi[a] - lo[a];
+                    bd.periodic[a] = 0; // the exchange supplies neighbours and periodic images
+                    offs.push_back(lo[a]);
+                }
+                descs.push_back(bd);
+            }
+    tsp_batch *b = nullptr;
+    st = create_batch(descs.data(), (int32_t)nboxes, stream, &b, offs.data(), d->n);
+    if (st != TSP_OK) return st;
+    b->level = true;
+    b->ldesc = *d;
+    b->box_lo = offs;
+    std::vector<int32_t> axis_box;
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < nb[a]; ++q)
+            for (int x = cuts[a][q]; x < cuts[a][q + 1]; ++x) axis_box.push_back(q);
+    hipError_t e = hipMalloc((void **)&b->d_axis_box, sizeof(int32_t) * axis_box.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_axis_box, axis_box.data(), sizeof(int32_t) * axis_box.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        tsp_batch_destroy(b);
+        return hip_fail(e, "level box table");
+    }
+    tsp::LevelArgs &L = b->largs;
+    L.inst = b->d_inst;
+    L.axis_box = b->d_axis_box;
+    for (int a = 0; a < 3; ++a) {
+        L.dom[a] = d->n[a];
+        L.nbox[a] = nb[a];
+        L.periodic[a] = d->periodic[a];
+    }
+    L.dirichlet = d->bc == TSP_BC_DIRICHLET;
+    L.G = d->ghost;
+    L.nc = system_ncomp(d->system);
+    L.nboxes = (int32_t)nboxes;
+    for (int c = 0; c < TSP_MAX_COMP; ++c) L.target[c] = d->target[c];
+    for (int i = 0; i < b->count; ++i) {
+        const tsp_desc &bd = b->d[i];
+        const long long g = (long long)(bd.n[0] + 2 * L.G) * (bd.n[1] + 2 * L.G) * (bd.n[2] + 2 * L.G) -
+                            (long long)bd.n[0] * bd.n[1] * bd.n[2];
+        b->max_shell = std::max(b->max_shell, g);
+    }
+    *out = b;
+    return TSP_OK;
+}
+
+tsp_status tsp_level_boxes(tsp_batch *b, int32_t *lo_hi) {
+    if (!b || !lo_hi) return fail(TSP_INVALID_ARG, "level or output is NULL");
+    if (!b->level) return fail(TSP_INVALID_ARG, "not a level (tsp_level_create)");
+    for (int i = 0; i < b->count; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo_hi[6 * i + a] = b->box_lo[3 * i + a];
+            lo_hi[6 * i + 3 + a] = b->box_lo[3 * i + a] + b->d[i].n[a] - 1;
+        }
+    return TSP_OK;
+}
+
+static tsp_status level_copy(tsp_batch *b, double *buf, int32_t on_device, bool scatter) {
+    if (!b || !buf) return fail(TSP_INVALID_ARG, "level or buffer is NULL");
+    if (!b->level) return fail(TSP_INVALID_ARG, "not a level (tsp_level_create)");
+    const size_t bytes =
+        sizeof(double) * (size_t)b->ldesc.n[0] * b->ldesc.n[1] * b->ldesc.n[2] * system_ncomp(b->ldesc.system);
+    double *dbuf = buf;
+    double *staging = nullptr;
+    if (!on_device) {
+        TSP_HIP(hipMalloc((void **)&staging, bytes));
+        dbuf = staging;
+    }
+    hipError_t e = hipSuccess;
+    if (scatter && staging) e = hipMemcpyAsync(staging, buf, bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = tsp::launch_level_copy(b->largs, b->role[TSP_ROLE_OLD], dbuf, scatter, b->stream);
+    if (e == hipSuccess && !scatter && staging) e = hipMemcpyAsync(buf, staging, bytes, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) return hip_fail(e, scatter ? "level set_state" : "level get_state");
+    return TSP_OK;
+}
+
+tsp_status tsp_level_set_state(tsp_batch *b, const double *src, int32_t on_device) {
+    if (!b || !b->level) return fail(TSP_INVALID_ARG, "not a level (tsp_level_create)");
+    // initialise(): the clock and the register roles restart
+    std::memcpy(b->role, kIdentityRoles, sizeof(b->role));
+    b->steps = 0;
+    b->time.assign(b->count, 0.0);
+    return level_copy(b, const_cast<double *>(src), on_device, true);
+}
+
+tsp_status tsp_level_get_state(tsp_batch *b, double *dst, int32_t on_device) {
+    return level_copy(b, dst, on_device, false);
+}
+
 } // extern "C"
 
 // ===========================================================================
@@ -519,6 +732,8 @@ struct tsp_batch_adjoint {
     int *d_counter = nullptr;       // one trace row counter per instance
     double *d_ckpt = nullptr;       // [slot][instance state]
     int arole[4] = {0, 1, 2, 3};
+    int roles0[4] = {0, 1, 2, 3};  // the batch's register roles when the forward pass started
+    int groles0[4] = {-1, -1, -1, -1}; // roles0 the forward graphs were captured for
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
     tsp_batch_cfg gcfg{-1, -1, -1, -1, -1, -1};
@@ -620,7 +835,7 @@ tsp_status batch_run_forward_range(tsp_batch_adjoint *a, int n0, int n1) {
     int n = n0;
     while (n < n1) {
         if (n1 - n >= 2) {
-            const int parity = (int)(b->steps & 1);
+            const int parity = (int)((b->steps - a->step0) & 1); // roles are roles0 after an even count
             tsp_status st =
                 batch_capture_pair(b->stream, &a->fgraph[parity], b->role, [&] { return enqueue_batch_fwd_gather(a); });
             if (st != TSP_OK) return st;
@@ -678,7 +893,8 @@ tsp_status batch_save_checkpoint(tsp_batch_adjoint *a, int slot) {
 tsp_status batch_restore_checkpoint(tsp_batch_adjoint *a, int slot, int step, const std::vector<double> &time) {
     tsp_batch *b = a->b;
     b->steps = a->step0 + step;
-    std::memcpy(b->role, (b->steps & 1) ? kOddRoles : kEvenRoles, sizeof(b->role));
+    if (step & 1) rk4_next_roles(a->roles0, b->role);
+    else std::memcpy(b->role, a->roles0, sizeof(b->role));
     b->time = time;
     for (int i = 0; i < b->count; ++i) {
         const tsp_desc &d = b->d[i];
@@ -775,6 +991,11 @@ tsp_status tsp_batch_adjoint_forward(tsp_batch_adjoint *a) {
     if (st != TSP_OK) return st;
     a->step0 = b->steps;
     a->time0 = b->time;
+    std::memcpy(a->roles0, b->role, sizeof(a->roles0));
+    if (std::memcmp(a->groles0, a->roles0, sizeof(a->roles0)) != 0) { // forward graphs belong to other roles
+        drop_batch_adjoint_graphs(a);
+        std::memcpy(a->groles0, a->roles0, sizeof(a->roles0));
+    }
     if ((st = batch_save_checkpoint(a, 0)) != TSP_OK) return st;
     TSP_HIP(tsp::launch_batch_set_counter(a->d_counter, b->count, 0, b->stream));
     int n = 0;
@@ -806,7 +1027,7 @@ tsp_status tsp_batch_adjoint_backward(tsp_batch_adjoint *a, const double *dtrace
     if (st != TSP_OK) return st;
     TSP_HIP(hipMemcpyAsync(a->d_dtrace, dtrace, sizeof(double) * (size_t)b->count * a->nsteps * a->nrec,
                            dtrace_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, b->stream));
-    std::memcpy(a->arole, kEvenRoles, sizeof(a->arole));
+    std::memcpy(a->arole, kIdentityRoles, sizeof(a->arole));
     for (int i = 0; i < b->count; ++i) // lambda = 0 in adjoint register 0 (ghosts stay zero)
         TSP_HIP(tsp::launch_zero(a->apool + b->reg_off[4 * i], b->L[i].elems, b->stream));
     int bparity = 0;

@@ -22,6 +22,7 @@ struct BatchInstance {
     int32_t variant;         // index into the batched (stencil, KO order) list
     int32_t periodic[3];
     int32_t dirichlet, abl_T, zchunk;
+    int32_t off[3], dom[3]; // a box of a level: its offset in the domain and the domain's cells per axis
     double dt;
     double gf1[3], gf2[3];
     double ko_strength, gamma;
@@ -107,6 +108,10 @@ template <class Sys, int N, int R, int TY, int RS, int SM = -1, bool ADJ = false
             a.periodic[2] = I.periodic[2];
             a.dirichlet = I.dirichlet;
             a.abl_T = I.abl_T;
+            for (int d = 0; d < 3; ++d) {
+                a.off[d] = I.off[d];
+                a.dom[d] = I.dom[d];
+            }
             a.zchunk = I.zchunk;
             a.tiles_x = a.tiles_y = a.nzc = 0;
             a.xcd_remap = 0;
@@ -229,5 +234,21 @@ hipError_t launch_batch_gather_trace(const BatchInstance *inst, int count, int r
 hipError_t launch_batch_inject_trace(const BatchInstance *inst, int count, int role, const int *rec_off, int nrec,
                                      const double *dtrace, int nrows, int *counter, hipStream_t stream);
 hipError_t launch_batch_set_counter(int *counter, int count, int v, hipStream_t stream);
+
+// A level split into boxes (tsp_level_*): the instances of a batch are its
+// boxes, instance b = bx + nbox[0] * (by + nbox[1] * bz) (level.hip).
+struct LevelArgs {
+    const BatchInstance *inst;
+    const int32_t *axis_box; // box index along axis a of every domain index: [dom0 | dom1 | dom2]
+    int32_t dom[3], nbox[3], periodic[3];
+    int32_t dirichlet, G, nc, nboxes;
+    double target[TSP_MAX_COMP];
+};
+// FillPatch of register `role` of every box: ghost cells from the owning box's
+// valid cells (periodic images along periodic axes), Dirichlet values outside
+// non-periodic faces; `max_shell` is the ghost-cell count of the largest box
+hipError_t launch_level_exchange(const LevelArgs &L, int role, long long max_shell, hipStream_t stream);
+// the domain's dense valid state into (scatter) or out of the boxes' register `role`
+hipError_t launch_level_copy(const LevelArgs &L, int role, double *dense, bool scatter, hipStream_t stream);
 
 } // namespace tsp

@@ -163,6 +163,10 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     for (int i = 0; i < 3; ++i) a.periodic[i] = s->d.periodic[i];
     a.dirichlet = s->d.bc == TSP_BC_DIRICHLET;
     a.abl_T = s->d.abl_thickness;
+    for (int i = 0; i < 3; ++i) {
+        a.off[i] = 0;
+        a.dom[i] = s->d.n[i];
+    }
     a.zchunk = s->cfg.zchunk;
     a.xcd_remap = s->cfg.xcd_remap;
     a.zalt = s->cfg.zalt;
@@ -934,6 +938,8 @@ struct tsp_adjoint {
     int64_t state_elems = 0;
     double *abase[4] = {nullptr, nullptr, nullptr, nullptr};
     int arole[4] = {0, 1, 2, 3};
+    int roles0[4] = {0, 1, 2, 3};      // the solver's register roles when the forward pass started
+    int groles0[4] = {-1, -1, -1, -1}; // roles0 the forward graphs were captured for
     tsp::LaunchFn launch = nullptr;
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
@@ -1038,7 +1044,7 @@ tsp_status run_forward_range(tsp_adjoint *a, int n0, int n1) {
     int n = n0;
     while (n < n1) {
         if (n1 - n >= 2) {
-            const int parity = (int)(s->steps & 1);
+            const int parity = (int)((s->steps - a->step0) & 1); // roles are roles0 after an even count
             tsp_status st = capture_pair(s->stream, &a->fgraph[parity], s->role, [&] { return enqueue_fwd_gather(a); });
             if (st != TSP_OK) return st;
             TSP_HIP(hipGraphLaunch(a->fgraph[parity], s->stream));
@@ -1085,11 +1091,14 @@ tsp_status save_checkpoint(tsp_adjoint *a, int slot) {
 // put checkpoint `slot` (the state `step` steps into the forward pass) back into the forward solver
 tsp_status restore_checkpoint(tsp_adjoint *a, int slot, int step, double time) {
     tsp_solver *s = a->s;
-    // register roles are a function of the step parity (identity after an even
-    // number of steps since tsp_create / tsp_set_state)
-    const int even[4] = {0, 1, 2, 3}, odd[4] = {1, 0, 3, 2};
+    // the roles `step` RK4 steps after the start of the forward pass: an odd
+    // count swaps old <-> new and arg <-> arg_next
     s->steps = a->step0 + step;
-    std::memcpy(s->role, (s->steps & 1) ? odd : even, sizeof(s->role));
+    std::memcpy(s->role, a->roles0, sizeof(s->role));
+    if (step & 1) {
+        std::swap(s->role[TSP_ROLE_OLD], s->role[TSP_ROLE_NEW]);
+        std::swap(s->role[TSP_ROLE_ARG], s->role[TSP_ROLE_ARG_NEXT]);
+    }
     s->time = time;
     TSP_HIP(tsp::launch_unpack(reg_origin(s, s->role[TSP_ROLE_OLD]), a->d_ckpt + (int64_t)slot * a->state_elems,
                                s->d.n[0], s->d.n[1], s->d.n[2], s->nc, (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
@@ -1169,6 +1178,15 @@ tsp_status tsp_adjoint_forward(tsp_adjoint *a) {
     // checkpoints and trace rows count from the state at the start of the pass
     a->step0 = s->steps;
     a->time0 = s->time;
+    std::memcpy(a->roles0, s->role, sizeof(a->roles0));
+    if (std::memcmp(a->groles0, a->roles0, sizeof(a->roles0)) != 0) { // forward graphs belong to other roles
+        for (auto *g : {&a->fgraph[0], &a->fgraph[1]})
+            if (*g) {
+                (void)hipGraphExecDestroy(*g);
+                *g = nullptr;
+            }
+        std::memcpy(a->groles0, a->roles0, sizeof(a->roles0));
+    }
     tsp_status st = save_checkpoint(a, 0);
     if (st != TSP_OK) return st;
     TSP_HIP(tsp::launch_set_counter(a->d_counter, 0, s->stream));

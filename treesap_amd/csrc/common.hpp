@@ -116,6 +116,8 @@ struct StageArgs {
     int32_t periodic[3];
     int32_t dirichlet;
     int32_t abl_T;           // ABL thickness (0: no ABL decorator)
+    int32_t off[3];          // global index of the box's cell (0,0,0) within the domain (a box of a level)
+    int32_t dom[3];          // cells per axis of the domain (the absorbing layer is measured against it)
     int32_t zchunk;
     int32_t tiles_x, tiles_y, nzc;
     int32_t xcd_remap;

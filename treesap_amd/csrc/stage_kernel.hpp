@@ -417,9 +417,10 @@ struct FusedStage {
         // ABL damping: the x/y part of the squared distance to the bulk box is fixed per lane
         int cxy2 = 0;
         const int T = a.abl_T;
-        if (T > 0) {
-            const int dxv = gi - min(max(gi, T), a.nx - 1 - T);
-            const int dyv = gj - min(max(gj, T), a.ny - 1 - T);
+        if (T > 0) { // in domain coordinates (a box of a level sits at a.off)
+            const int xg = gi + a.off[0], yg = gj + a.off[1];
+            const int dxv = xg - min(max(xg, T), a.dom[0] - 1 - T);
+            const int dyv = yg - min(max(yg, T), a.dom[1] - 1 - T);
             cxy2 = dxv * dxv + dyv * dyv;
         }
 
@@ -525,7 +526,8 @@ struct FusedStage {
                 });
                 double sigma = 0.0;
                 if (T > 0) {
-                    const int dzv = z - min(max(z, T), a.nz - 1 - T);
+                    const int zg = z + a.off[2];
+                    const int dzv = zg - min(max(zg, T), a.dom[2] - 1 - T);
                     const int d2 = cxy2 + dzv * dzv;
                     sigma = d2 == 0 ? 0.0 : sig[d2];
                 }
@@ -617,9 +619,10 @@ struct FusedStage {
 
         int cxy2 = 0;
         const int T = a.abl_T;
-        if (T > 0) {
-            const int dxv = gi - min(max(gi, T), a.nx - 1 - T);
-            const int dyv = gj - min(max(gj, T), a.ny - 1 - T);
+        if (T > 0) { // in domain coordinates (a box of a level sits at a.off)
+            const int xg = gi + a.off[0], yg = gj + a.off[1];
+            const int dxv = xg - min(max(xg, T), a.dom[0] - 1 - T);
+            const int dyv = yg - min(max(yg, T), a.dom[1] - 1 - T);
             cxy2 = dxv * dxv + dyv * dyv;
         }
 
@@ -717,7 +720,8 @@ struct FusedStage {
                 });
                 double sigma = 0.0;
                 if (T > 0) {
-                    const int dzv = z - min(max(z, T), a.nz - 1 - T);
+                    const int zg = z + a.off[2];
+                    const int dzv = zg - min(max(zg, T), a.dom[2] - 1 - T);
                     const int d2 = cxy2 + dzv * dzv;
                     sigma = d2 == 0 ? 0.0 : sig[d2];
                 }

@@ -280,3 +280,58 @@ class Batch:
 
     def sync(self) -> None:
         _lib.check(_lib.lib().tsp_batch_sync(self._h), "tsp_batch_sync")
+
+    def euler_steps(self, nsteps: int) -> None:
+        _lib.check(_lib.lib().tsp_batch_euler_steps(self._h, int(nsteps)), "tsp_batch_euler_steps")
+
+
+class Level(Batch):
+    """One level split into boxes of at most `block` cells per axis (the BoxArray of
+    SingleLevelIntegrator(model, block_size), SingleLevelIntegrator.hpp:47-49,172-176), each box its own
+    registers, ghosts exchanged between boxes before every stage; (phi, Pi) system. Results are
+    bit-identical to one box of the whole domain."""
+
+    def __init__(self, case: Case, block, stream=None):
+        self.case = case
+        self._desc = make_desc(case)
+        blk = (C.c_int32 * 3)(*[int(b) for b in block])
+        handle = C.c_void_p()
+        st = _lib.lib().tsp_level_create(C.byref(self._desc), blk, C.c_void_p(stream), C.byref(handle))
+        _lib.check(st, "tsp_level_create")
+        self._h = handle
+        n = C.c_int32()
+        _lib.check(_lib.lib().tsp_batch_size(self._h, C.byref(n), None), "tsp_batch_size")
+        lohi = (C.c_int32 * (6 * n.value))()
+        _lib.check(_lib.lib().tsp_level_boxes(self._h, lohi), "tsp_level_boxes")
+        self.boxes = [tuple(lohi[6 * b: 6 * b + 6]) for b in range(n.value)]
+        self.cases = [case.replace(n=tuple(hi - lo + 1 for lo, hi in zip(bx[:3], bx[3:])), periodic=(0, 0, 0))
+                      for bx in self.boxes]
+
+    def set_state(self, state) -> None:  # noqa: D401 -- the whole domain
+        """The domain's valid state (ncomp, nz, ny, nx) float64, numpy or torch CUDA."""
+        if _is_torch_cuda(state):
+            if tuple(state.shape) != self.case.shape or str(state.dtype) != "torch.float64":
+                raise ValueError(f"state must be float64 of shape {self.case.shape}")
+            state = state.contiguous()
+            _torch_ready(state)
+            _lib.check(_lib.lib().tsp_level_set_state(self._h, C.c_void_p(state.data_ptr()), 1), "tsp_level_set_state")
+            return
+        arr = np.ascontiguousarray(state, dtype=np.float64)
+        if arr.shape != self.case.shape:
+            raise ValueError(f"state must have shape {self.case.shape}, got {arr.shape}")
+        _lib.check(_lib.lib().tsp_level_set_state(self._h, arr.ctypes.data_as(C.c_void_p), 0), "tsp_level_set_state")
+
+    def get_state(self, out=None):
+        if out is not None and _is_torch_cuda(out):
+            if not out.is_contiguous() or tuple(out.shape) != self.case.shape or str(out.dtype) != "torch.float64":
+                raise ValueError("out must be a contiguous float64 tensor of the state shape")
+            _torch_ready(out)
+            _lib.check(_lib.lib().tsp_level_get_state(self._h, C.c_void_p(out.data_ptr()), 1), "tsp_level_get_state")
+            return out
+        arr = np.empty(self.case.shape, dtype=np.float64)
+        _lib.check(_lib.lib().tsp_level_get_state(self._h, arr.ctypes.data_as(C.c_void_p), 0), "tsp_level_get_state")
+        return arr
+
+    @property
+    def time(self) -> float:
+        return Batch.time(self, 0)
