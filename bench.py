@@ -118,20 +118,9 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     for i, (case, centre) in enumerate(params):
         b.set_state(i, gaussian_state_device(case, centre))
     torch.cuda.synchronize()
-    cfgs = [b.kernel]
-    if tune:
-        cfgs = [dict(tile_y=ty, zchunk=zc, grid_blocks=gb, zalt=1, halo_ring=hr, narrow_tiles=nt) for ty in (8, 16)
-                for zc in (0, 64, 128) for gb in (0, 1024) for hr in (0, 1) for nt in (0, 1)]
-
-    def timed(cfg, n):
-        b.kernel = cfg
-        b.graph_rk4_steps(2)
-        return b.time_rk4(n, use_graph=True) / n
-
-    # short timing of every candidate, then the three fastest again over more steps
-    ranked = sorted((timed(cfg, 6), i) for i, cfg in enumerate(cfgs))
-    best = min((timed(cfgs[i], 20), dict(cfgs[i])) for _, i in ranked[:3])
-    b.kernel = best[1]
+    # the library's batch autotuner (tsp_batch_autotune): every candidate configuration timed on
+    # this batch, the fastest kept; the state is restored
+    best = (None, b.autotune(6) if tune else b.kernel)
     b.graph_rk4_steps(2)  # capture the graph of the chosen configuration (an even count keeps the parity)
     for i, (case, centre) in enumerate(params):
         b.set_state(i, gaussian_state_device(case, centre))
@@ -168,6 +157,7 @@ def ensemble_leg(count: int, steps: int, tune: bool):
         "batched_s": batched_ms / 1e3,
         "plain_s": plain_ms / 1e3,
         "batch_kernel": best[1],
+        "batch_kernel_chosen_by": "tsp_batch_autotune" if tune else "default",
         "batched_equals_plain": "bitwise, every instance",
     }
 

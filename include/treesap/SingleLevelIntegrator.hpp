@@ -5,11 +5,13 @@
 // allocates its registers. Right-hand sides the library carries (registered, or
 // mapped by system_traits) run on its tiled stage kernels; any other functor
 // runs on a stage kernel compiled from it (device/generic.hpp), which needs
-// this program to be compiled by a HIP compiler. `block_size` is accepted for
-// source compatibility.
-// The reference splits the level into boxes of at most block_size cells, and
-// its results do not depend on that split; here the level is one box, and
-// the ghost exchange between boxes never arises.
+// this program to be compiled by a HIP compiler.
+// As in the reference, the level is split into boxes of at most block_size
+// cells per axis (SingleLevelIntegrator.hpp:47-49,172-176), each box with its
+// own registers, and FillPatch exchanges ghosts between the boxes before every
+// stage (tsp_level_*); results do not depend on the split. The (phi, Pi)
+// system runs split levels; the other systems run the level as one box, which
+// gives the same results.
 // initialise() evaluates the initial condition on the host at cell centres
 // x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93) and uploads it.
 // pre_step() exports a plotfile when one is due. The FillPatch before each
@@ -53,9 +55,7 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     static constexpr bool library_kernels = device::runs_library_kernels<PDEModelType>;
 
     SingleLevelIntegrator(PDEModelType model_, const amrex::IntVect &block_size)
-        : Base(std::move(model_)), solver(device::make_desc(model)) {
-        for (int a = 0; a < 3; ++a)
-            if (block_size[a] < 1) throw std::invalid_argument("treesap: block_size entries must be positive");
+        : Base(std::move(model_)), solver(make_solver(model, block_size)) {
         if constexpr (!library_kernels) generic.emplace(model.get_rhs(), model.get_geometry().data());
     }
     ~SingleLevelIntegrator() override = default;
@@ -141,6 +141,13 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     void require_model_dt(amrex::Real dt) const {
         if (!(dt == model.dt_hint()))
             throw std::invalid_argument("treesap: the device solver steps with the model's dt_hint()");
+    }
+
+    static device::Solver make_solver(const PDEModelType &m, const amrex::IntVect &block_size) {
+        for (int a = 0; a < 3; ++a)
+            if (block_size[a] < 1) throw std::invalid_argument("treesap: block_size entries must be positive");
+        const int block[3] = {block_size[0], block_size[1], block_size[2]};
+        return device::Solver(device::make_desc(m), block);
     }
 
     mutable device::Solver solver;

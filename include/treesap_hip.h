@@ -238,6 +238,14 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
 tsp_status tsp_batch_graph_prepare(tsp_batch *b);
 /* `nsteps` forward Euler steps of every instance (SingleLevelEulerIntegrator.hpp:66-88). */
 tsp_status tsp_batch_euler_steps(tsp_batch *b, int64_t nsteps);
+/* Time candidate batch configurations (tile height, z chunk, persistent grid,
+ * halo ring, narrow tiles) over `steps` graph-replayed RK4 steps each, the
+ * three fastest again over three times as many, and keep the fastest; every
+ * instance's state, the register roles and the clock are restored. Results
+ * are cached per (device arch, batch or level, instance count, cells, stencil
+ * width and KO order groups) in the file named by TSP_TUNE_CACHE, if set. The
+ * chosen configuration is written to *chosen if non-NULL. */
+tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen);
 
 /* A level split into boxes: replaces the BoxArray that
  * SingleLevelIntegrator(model, block_size) builds (SingleLevelIntegrator.hpp:47-49,

@@ -4,7 +4,7 @@
 //   desc.bin   the tsp_desc the model was mapped to (raw struct bytes)
 //   init.bin   the state after initialise(), a[((c*nz+k)*ny+j)*nx+i]
 //   final.bin  the state after the steps
-//   meta.txt   "<integrator> <steps> <time as %.17g>"
+//   meta.txt   "<integrator> <steps> <time as %.17g> <boxes of the level>"
 // The tests replay desc + init.bin on the CPU oracle and compare final.bin.
 //
 //   dropin_gpu <outdir> [scenario ...]
@@ -40,7 +40,8 @@ template <class Integ> static void dump(const fs::path &dir, Integ &integ, const
     write_bin(dir / "final.bin", fin.data(), fin.size() * sizeof(double));
     std::ofstream m(dir / "meta.txt");
     char buf[128];
-    std::snprintf(buf, sizeof buf, "%s %zu %.17g\n", kind, integ.get_steps(), integ.get_time());
+    std::snprintf(buf, sizeof buf, "%s %zu %.17g %d\n", kind, integ.get_steps(), integ.get_time(),
+                  integ.device_solver().boxes());
     m << buf;
 }
 
@@ -89,6 +90,24 @@ static void scenario_abl_ko(const fs::path &dir) {
     dump(dir, integ, u0, "rk4");
 }
 
+// (phi, Pi) wave with KO on a periodic ragged box split into boxes of at most
+// 7 x 9 x 64 cells (a level of 3 x 2 x 1 boxes), forward Euler then RK4 steps
+static void scenario_level_euler(const fs::path &dir) {
+    const int p[3] = {1, 1, 1};
+    const amrex::RealBox rb({0.0, 0.0, 0.0}, {1.0, 0.9, 0.8});
+    const amrex::Geometry geom(amrex::Box({0, 0, 0}, {19, 17, 15}), &rb, amrex::CoordSys::cartesian, p);
+    using RHSType = KODecorator<ScalarWave<2, 3>, 3>;
+    using InitType = FunctionInit<2>;
+    using ModelType = PDEModel<RHSType, InitType>;
+    InitType init({"sin(2*pi*x)*cos(2*pi*y/0.9) + 0.3*sin(2*pi*z/0.8)", "0.5*cos(2*pi*(x + y))"});
+    ModelType model({"phi", "Pi"}, geom, 0.002, RHSType{{}, 0.1}, init);
+    SingleLevelEulerIntegrator<ModelType> integ(std::move(model), {7, 9, 64});
+    integ.initialise();
+    const auto u0 = integ.get_state();
+    for (int s = 0; s < 6; ++s) integ.step();
+    dump(dir, integ, u0, "euler");
+}
+
 // nonlinear wave with a workspace temporary, KO, periodic ragged box, forward Euler
 static void scenario_nonlinear_euler(const fs::path &dir) {
     const int p[3] = {1, 1, 1};
@@ -135,7 +154,7 @@ int main(int argc, char **argv) {
     }
     const fs::path out = argv[1];
     std::vector<std::string> which(argv + 2, argv + argc);
-    if (which.empty()) which = {"first_order", "abl_ko", "nonlinear_euler", "tensor"};
+    if (which.empty()) which = {"first_order", "abl_ko", "nonlinear_euler", "tensor", "level_euler"};
     for (const auto &w : which) {
         const fs::path dir = out / w;
         fs::create_directories(dir);
@@ -146,6 +165,7 @@ int main(int argc, char **argv) {
             else if (w == "abl_ko") scenario_abl_ko(dir);
             else if (w == "nonlinear_euler") scenario_nonlinear_euler(dir);
             else if (w == "tensor") scenario_tensor(dir);
+            else if (w == "level_euler") scenario_level_euler(dir);
             else {
                 std::fprintf(stderr, "unknown scenario %s\n", w.c_str());
                 return 2;

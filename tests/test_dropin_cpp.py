@@ -145,7 +145,9 @@ def test_registered_rhs_matches_oracle(tmp_path, case_id, geometry):
 
 # ---- whole programs on the GPU ------------------------------------------------------
 
-SCENARIOS = ["first_order", "abl_ko", "nonlinear_euler", "tensor"]
+SCENARIOS = ["first_order", "abl_ko", "nonlinear_euler", "tensor", "level_euler"]
+# boxes of the level each program's block size gives: (phi, Pi) models are split
+LEVEL_BOXES = {"first_order": 1, "abl_ko": 8, "nonlinear_euler": 1, "tensor": 1, "level_euler": 6}
 
 
 def _read_desc(path: Path) -> _lib.TspDesc:
@@ -169,8 +171,9 @@ def gpu_runs(tmp_path_factory):
 def test_dropin_program_matches_oracle(gpu_runs, scenario):
     d = gpu_runs / scenario
     case = case_from_desc(_read_desc(d / "desc.bin"))
-    kind, steps, time = (d / "meta.txt").read_text().split()
+    kind, steps, time, boxes = (d / "meta.txt").read_text().split()
     steps = int(steps)
+    assert int(boxes) == LEVEL_BOXES[scenario]
     init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
     final = np.fromfile(d / "final.bin", dtype=np.float64).reshape(case.shape)
     o = orc.OracleSolver(case)
@@ -204,6 +207,10 @@ INIT_EXPRS = {
                          + 0.3 * math.sin(2 * math.pi * z / 0.8),
                          lambda x, y, z: 0.5 * math.cos(2 * math.pi * (x + y))]),
     "tensor": ((0.0,) * 3, (1.0,) * 3, (24, 24, 24), [_tensor_init(c) for c in range(12)]),
+    "level_euler": ((0.0,) * 3, (1.0, 0.9, 0.8), (20, 18, 16),
+                    [lambda x, y, z: math.sin(2 * math.pi * x) * math.cos(2 * math.pi * y / 0.9)
+                     + 0.3 * math.sin(2 * math.pi * z / 0.8),
+                     lambda x, y, z: 0.5 * math.cos(2 * math.pi * (x + y))]),
 }
 
 

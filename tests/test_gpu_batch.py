@@ -157,3 +157,28 @@ def test_random_batches_match_oracle(seed):
         o.set_state(st)
         o.rk4_steps(3)
         assert np.array_equal(b.get_state(i), o.get_state()), (i, case.n)
+
+
+def test_batch_autotune_restores_state_and_keeps_results():
+    """tsp_batch_autotune times its candidates on the batch itself: afterwards every instance's
+    state, the step count and the clock are what they were, and stepping with the chosen
+    configuration equals the oracle."""
+    _say("[batch] autotune")
+    cases = mixed_cases()
+    b = Batch([c for c, _ in cases])
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.rk4_steps(1)
+    before = [b.get_state(i) for i in range(len(cases))]
+    t_before = [b.time(i) for i in range(len(cases))]
+    cfg = b.autotune(2)
+    assert cfg == b.kernel and cfg["tile_y"] in (8, 16)
+    assert b.steps == 1 and [b.time(i) for i in range(len(cases))] == t_before
+    for i in range(len(cases)):
+        assert np.array_equal(b.get_state(i), before[i]), i
+    b.graph_rk4_steps(2)
+    for i, (case, st) in enumerate(cases):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(3)
+        assert np.array_equal(b.get_state(i), o.get_state()), i

@@ -127,6 +127,7 @@ PROTOTYPES = {
     "tsp_batch_graph_rk4_steps": (C.c_int, [_VP, C.c_int64]),
     "tsp_batch_graph_prepare": (C.c_int, [_VP]),
     "tsp_batch_euler_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_batch_autotune": (C.c_int, [_VP, C.c_int32, C.POINTER(TspBatchCfg)]),
     "tsp_level_create": (C.c_int, [C.POINTER(TspDesc), C.POINTER(C.c_int32), _VP, C.POINTER(_VP)]),
     "tsp_level_boxes": (C.c_int, [_VP, C.POINTER(C.c_int32)]),
     "tsp_level_set_state": (C.c_int, [_VP, _VP, C.c_int32]),

@@ -12,6 +12,8 @@
 #include "capi_internal.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -572,6 +574,167 @@ tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, d
 tsp_status tsp_batch_sync(tsp_batch *b) {
     if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
     TSP_HIP(hipStreamSynchronize(b->stream));
+    return TSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner of the batched kernels
+// ---------------------------------------------------------------------------
+
+// cache key: device arch, level or batch, instance count, cells, and the
+// (stencil width, KO order) groups present
+static std::string batch_tune_key(const tsp_batch *b) {
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    std::string arch = "unknown";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+    for (char &c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    int64_t cells = 0;
+    for (const tsp_desc &d : b->d) cells += (int64_t)d.n[0] * d.n[1] * d.n[2];
+    std::string groups;
+    for (int v = 0; v < tsp::kBatchVariants; ++v)
+        if (b->group_n[v] > 0) groups += "n" + std::to_string(tsp::kBatchN[v]) + "r" + std::to_string(tsp::kBatchR[v]);
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s|%s|%d|%lld|%s", arch.c_str(), b->level ? "level" : "batch", b->count,
+                  (long long)cells, groups.c_str());
+    return buf;
+}
+
+static bool batch_cache_lookup(const std::string &key, tsp_batch_cfg *cfg) {
+    const char *path = std::getenv("TSP_TUNE_CACHE");
+    if (!path || !*path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        char k[256];
+        tsp_batch_cfg c{};
+        double ms = 0;
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d %lf", k, &c.tile_y, &c.zchunk, &c.grid_blocks, &c.zalt,
+                        &c.halo_ring, &c.narrow_tiles, &ms) == 8 &&
+            key == k && validate_batch_cfg(&c) == TSP_OK) {
+            *cfg = c;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+static void batch_cache_store(const std::string &key, const tsp_batch_cfg &c, double ms) {
+    const char *path = std::getenv("TSP_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), c.tile_y, c.zchunk, c.grid_blocks, c.zalt, c.halo_ring,
+                 c.narrow_tiles, ms);
+    std::fclose(f);
+}
+
+tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    if (steps < 2) steps = 2;
+    steps += steps & 1; // whole graph replays
+    tsp_status st = build_work(b);
+    if (st != TSP_OK) return st;
+    const std::string key = batch_tune_key(b);
+    tsp_batch_cfg best = b->cfg;
+    if (batch_cache_lookup(key, &best)) {
+        b->cfg = best;
+        if (chosen) *chosen = best;
+        return build_work(b);
+    }
+    // save the state: roles, clock and every instance's state_old
+    int saved_role[4];
+    std::memcpy(saved_role, b->role, sizeof(saved_role));
+    const int64_t saved_steps = b->steps;
+    const std::vector<double> saved_time = b->time;
+    const tsp_batch_cfg saved_cfg = b->cfg;
+    int64_t backup_elems = 0;
+    for (int i = 0; i < b->count; ++i) backup_elems += b->L[i].elems;
+    double *backup = nullptr;
+    TSP_HIP(hipMalloc((void **)&backup, sizeof(double) * backup_elems));
+    auto copy_old = [&](bool save) -> hipError_t {
+        int64_t off = 0;
+        for (int i = 0; i < b->count; ++i) {
+            double *reg = b->pool + b->reg_off[4 * i + saved_role[TSP_ROLE_OLD]];
+            hipError_t e = save ? hipMemcpyAsync(backup + off, reg, sizeof(double) * b->L[i].elems,
+                                                 hipMemcpyDeviceToDevice, b->stream)
+                                : hipMemcpyAsync(reg, backup + off, sizeof(double) * b->L[i].elems,
+                                                 hipMemcpyDeviceToDevice, b->stream);
+            if (e != hipSuccess) return e;
+            off += b->L[i].elems;
+        }
+        return hipSuccess;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = copy_old(true);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) st = hip_fail(e, "batch autotune setup");
+
+    std::vector<tsp_batch_cfg> cands;
+    for (int ty : {8, 16})
+        for (int zc : {0, 64, 128})
+            for (int gb : {0, 1024})
+                for (int hr = 0; hr < 2; ++hr)
+                    for (int nt = 0; nt < 2; ++nt) cands.push_back(tsp_batch_cfg{ty, zc, gb, 1, hr, nt});
+    // mean ms per step of `c` over n graph-replayed steps, from the saved state
+    auto time_cfg = [&](const tsp_batch_cfg &c, int n, double *ms) -> tsp_status {
+        b->cfg = c;
+        tsp_status r = build_work(b);
+        if (r == TSP_OK) r = tsp_batch_graph_rk4_steps(b, 2); // capture and warm up
+        if (r != TSP_OK) return r;
+        if ((e = hipEventRecord(e0, b->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((r = tsp_batch_graph_rk4_steps(b, n)) != TSP_OK) return r;
+        if ((e = hipEventRecord(e1, b->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        float f = 0.f;
+        if ((e = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+        *ms = f / n;
+        // restart from the saved state so values stay bounded
+        std::memcpy(b->role, saved_role, sizeof(saved_role));
+        b->steps = saved_steps;
+        b->time = saved_time;
+        e = copy_old(false);
+        return e == hipSuccess ? TSP_OK : hip_fail(e, "batch autotune restore");
+    };
+    // every candidate over `steps` steps, then the three fastest over three times as many
+    std::vector<std::pair<double, int>> ranked;
+    for (size_t i = 0; i < cands.size() && st == TSP_OK; ++i) {
+        double ms = 0.0;
+        st = time_cfg(cands[i], steps, &ms);
+        ranked.push_back({ms, (int)i});
+    }
+    double best_ms = 1e300;
+    if (st == TSP_OK) {
+        std::sort(ranked.begin(), ranked.end());
+        for (size_t q = 0; q < std::min<size_t>(3, ranked.size()) && st == TSP_OK; ++q) {
+            double ms = 0.0;
+            st = time_cfg(cands[ranked[q].second], 3 * steps, &ms);
+            if (st == TSP_OK && ms < best_ms) {
+                best_ms = ms;
+                best = cands[ranked[q].second];
+            }
+        }
+    }
+    // restore (the state is already restored after every candidate)
+    std::memcpy(b->role, saved_role, sizeof(saved_role));
+    b->steps = saved_steps;
+    b->time = saved_time;
+    e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess && st == TSP_OK) st = hip_fail(e, "batch autotune");
+    (void)hipFree(backup);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    b->cfg = st == TSP_OK ? best : saved_cfg;
+    tsp_status st2 = build_work(b);
+    if (st != TSP_OK) return st;
+    if (st2 != TSP_OK) return st2;
+    batch_cache_store(key, best, best_ms);
+    if (chosen) *chosen = best;
     return TSP_OK;
 }
 

@@ -284,6 +284,12 @@ class Batch:
     def euler_steps(self, nsteps: int) -> None:
         _lib.check(_lib.lib().tsp_batch_euler_steps(self._h, int(nsteps)), "tsp_batch_euler_steps")
 
+    def autotune(self, steps: int = 6) -> dict:
+        """The library's batch autotuner (tsp_batch_autotune); returns the chosen configuration."""
+        cfg = _lib.TspBatchCfg()
+        _lib.check(_lib.lib().tsp_batch_autotune(self._h, int(steps), C.byref(cfg)), "tsp_batch_autotune")
+        return cfg.as_dict()
+
 
 class Level(Batch):
     """One level split into boxes of at most `block` cells per axis (the BoxArray of
