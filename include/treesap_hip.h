@@ -153,6 +153,23 @@ tsp_status tsp_get_register(tsp_solver *s, int32_t role, double *dst_valid_plus_
 /* Replaces `nsteps` calls of Integrator::step() (Integrator.hpp:91-106) on
  * SingleLevelRK4Integrator::computeNewState (SingleLevelRK4Integrator.hpp:95-157). */
 tsp_status tsp_rk4_steps(tsp_solver *s, int64_t nsteps);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). `nsteps` RK4 steps of `state_in` into `state_out` (both
+ * device pointers, the valid layout of tsp_set_state) with the solver's model,
+ * kernels and configuration, enqueued on the solver's stream; bit-identical
+ * to tsp_set_state + tsp_rk4_steps + tsp_get_state. The four RK4 registers
+ * live in a file-scope device array of TSP_NOALLOC_SCRATCH_BYTES per device
+ * instead of the solver's: the call allocates nothing, and a solver whose
+ * registers need more is rejected with TSP_INVALID_ARG. The solver's own
+ * registers, roles and clock are untouched. Every call on one device shares
+ * that device's array (it keeps the last call's contents and is cleared at
+ * the start of each call), so calls that overlap on one device race: order
+ * them on one stream or synchronise between them. The current device must be
+ * the one the solver was created on. */
+#define TSP_NOALLOC_SCRATCH_BYTES (128ull << 20)
+tsp_status tsp_rk4_steps_noalloc(tsp_solver *s, const double *state_in, double *state_out, int64_t nsteps);
 /* Same on SingleLevelEulerIntegrator::computeNewState (SingleLevelEulerIntegrator.hpp:66-88). */
 tsp_status tsp_euler_steps(tsp_solver *s, int64_t nsteps);
 

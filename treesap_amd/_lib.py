@@ -103,6 +103,7 @@ PROTOTYPES = {
     "tsp_fill_ghosts": (C.c_int, [_VP, C.c_int32]),
     "tsp_get_register": (C.c_int, [_VP, C.c_int32, _VP]),
     "tsp_rk4_steps": (C.c_int, [_VP, C.c_int64]),
+    "tsp_rk4_steps_noalloc": (C.c_int, [_VP, _VP, _VP, C.c_int64]),
     "tsp_euler_steps": (C.c_int, [_VP, C.c_int64]),
     "tsp_get_time": (C.c_int, [_VP, _DP, _I64P]),
     "tsp_get_kernel": (C.c_int, [_VP, C.POINTER(TspKernelCfg)]),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -40,6 +41,7 @@ struct tsp_solver {
     int role[4] = {0, 1, 2, 3}; // role -> register index
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int device = -1; // the device current at creation
     // TSP_POINTER_ACCESS set at creation: run the pointer-access stage kernel
     // that registers of 4 GiB or more use, on any size (for testing it)
     bool ptr_access = false;
@@ -381,6 +383,7 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
     for (int st = 0; st < 4; ++st) s->launch_stage[st] = lookup_stage_launcher(d->system, d->stencil_n, d->ko_order, st);
     if (d->system == TSP_SYS_WAVE_PHIPI) s->launch_pair = tsp::lookup_pair_phipi(d->stencil_n, d->ko_order);
     s->cfg = default_cfg(*d);
+    if (hipGetDevice(&s->device) != hipSuccess) s->device = -1;
     s->ptr_access = std::getenv("TSP_POINTER_ACCESS") != nullptr;
     // Geometry::CellSize = (prob_hi - prob_lo) / n_cells; grid factors as the stencils form them
     for (int a = 0; a < 3; ++a) {
@@ -514,6 +517,77 @@ tsp_status tsp_get_register(tsp_solver *s, int32_t role, double *dst) {
     TSP_HIP(hipStreamSynchronize(s->stream));
     TSP_HIP(hipMemcpy2D(dst, sizeof(double) * gx, src, sizeof(double) * s->pitch, sizeof(double) * gx, rows,
                         hipMemcpyDeviceToHost));
+    return TSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RK4 steps without any allocation: the four registers in a file-scope device
+// array (one copy per device, shared by every call on that device)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = size_t(128) << 20; // TSP_NOALLOC_SCRATCH_BYTES
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+tsp_status scratch_of_current_device(double **out) {
+    int dev = -1;
+    TSP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TSP_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return TSP_OK;
+}
+} // namespace
+
+tsp_status tsp_rk4_steps_noalloc(tsp_solver *s, const double *state_in, double *state_out, int64_t nsteps) {
+    if (!s || !state_in || !state_out) return fail(TSP_INVALID_ARG, "solver, input or output is NULL");
+    if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    if (!s->launch) return fail(TSP_UNSUPPORTED, "a custom model steps through tsp_rk4_steps_custom");
+    const int64_t reg = (s->reg_elems + 31) / 32 * 32; // 256-byte aligned registers
+    if (4 * reg * (int64_t)sizeof(double) > (int64_t)kScratchBytes)
+        return fail(TSP_INVALID_ARG, "the four registers of this solver (" +
+                                         std::to_string(4 * reg * (int64_t)sizeof(double)) +
+                                         " bytes) exceed the " + std::to_string(kScratchBytes) +
+                                         "-byte scratch of tsp_rk4_steps_noalloc");
+    int dev = -1;
+    TSP_HIP(hipGetDevice(&dev));
+    if (dev != s->device) return fail(TSP_INVALID_ARG, "the solver was created on another device than the current one");
+    double *ptr = nullptr;
+    tsp_status st = scratch_of_current_device(&ptr);
+    if (st != TSP_OK) return st;
+    // a stack copy of the solver stepping the scratch registers: same descriptor,
+    // kernels and configuration; the solver's own registers, roles and clock are untouched
+    tsp_solver t = *s;
+    t.graph[0] = t.graph[1] = nullptr;
+    for (int r = 0; r < 4; ++r) {
+        t.base[r] = ptr + r * reg;
+        t.role[r] = r;
+    }
+    // the scratch holds the last call's contents: zero every register (the NoOp
+    // ghosts are read as zero, as in a new solver), write the Dirichlet ghosts,
+    // then the caller's state
+    for (int r = 0; r < 4; ++r) {
+        TSP_HIP(tsp::launch_zero(t.base[r], s->reg_elems, s->stream));
+        TSP_HIP(tsp::launch_ghost_fill(ghost_args(&t, r), s->stream));
+    }
+    TSP_HIP(tsp::launch_unpack(reg_origin(&t, t.role[TSP_ROLE_OLD]), state_in, s->d.n[0], s->d.n[1], s->d.n[2], s->nc,
+                               (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        st = enqueue_rk4_step(&t, nullptr);
+        if (st != TSP_OK) return st;
+    }
+    TSP_HIP(tsp::launch_pack(state_out, reg_origin(&t, t.role[TSP_ROLE_OLD]), s->d.n[0], s->d.n[1], s->d.n[2], s->nc,
+                             (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
     return TSP_OK;
 }
 
