@@ -1,7 +1,9 @@
 // Integrator.hpp -- base of the time integrators: owns the model, counts
 // steps and accumulates time (Integrator.hpp:53-148). step() runs
 // pre_step(), then dt = step_impl(), steps += 1, time += dt, then post_step(),
-// each inside a ScopedTimer; initialise() runs initialise_impl().
+// each inside a ScopedTimer; initialise() runs initialise_impl(). The timers
+// record on the stream timing_stream() names (the device solver's stream in
+// the single-level integrators).
 #pragma once
 
 #include "pde-model/PDEModelConcepts.hpp"
@@ -27,11 +29,13 @@ class Integrator {
     virtual ~Integrator() = default;
 
     void initialise() {
+        ScopedTimer::StreamGuard on(timing_stream());
         ScopedTimer timer("Integrator::initialise()");
         initialise_impl();
     }
 
     void step() {
+        ScopedTimer::StreamGuard on(timing_stream());
         {
             ScopedTimer timer("Integrator::step() [PRE]");
             pre_step();
@@ -58,6 +62,8 @@ class Integrator {
     virtual amrex::Real step_impl() = 0;
     virtual void pre_step() {}
     virtual void post_step() {}
+    // hipStream_t of the integrator's device work (nullptr: none)
+    virtual void *timing_stream() const { return nullptr; }
 
     PDEModelType model;
     std::size_t steps = 0;

@@ -12,8 +12,10 @@
 // stage (tsp_level_*); results do not depend on the split. The (phi, Pi)
 // system runs split levels; the other systems run the level as one box, which
 // gives the same results.
-// initialise() evaluates the initial condition on the host at cell centres
-// x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93) and uploads it.
+// initialise() evaluates the initial condition at cell centres
+// x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93): a FunctionInit
+// on the device (its expressions compiled to postfix programs), any other
+// initial-condition functor on the host, uploaded.
 // pre_step() exports a plotfile when one is due. The FillPatch before each
 // stage (SingleLevelIntegrator.hpp:153-163) and the swap of old and new state
 // after the step happen inside the library's step.
@@ -28,6 +30,7 @@
 #include "device/generic_runner.hpp"
 #include "device/model_map.hpp"
 #include "device/solver.hpp"
+#include "init/FunctionInit.hpp"
 #include "util/PlotExporter.hpp"
 
 #include <cstdint>
@@ -57,8 +60,11 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     SingleLevelIntegrator(PDEModelType model_, const amrex::IntVect &block_size)
         : Base(std::move(model_)), solver(make_solver(model, block_size)) {
         if constexpr (!library_kernels) generic.emplace(model.get_rhs(), model.get_geometry().data());
+        ScopedTimer::set_timing_stream(solver.stream());
     }
-    ~SingleLevelIntegrator() override = default;
+    ~SingleLevelIntegrator() override {
+        if (ScopedTimer::timing_stream() == solver.stream()) ScopedTimer::set_timing_stream(nullptr);
+    }
 
     void configureExporter(const fs::path &path, int export_interval) {
         exporter.emplace([this](std::vector<amrex::Real> &v) { v = get_state(); }, model.get_geometry(),
@@ -69,6 +75,16 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     void initialise_impl() override {
         const InitType init = model.get_init();
         const amrex::Geometry &geom = model.get_geometry();
+        if constexpr (is_function_init<InitType>::value) {
+            // the expressions run on the device at every cell centre
+            std::vector<tsp_expr_op> ops;
+            std::vector<std::int32_t> start;
+            init.program(ops, start);
+            const amrex::Box &dom = geom.Domain();
+            const std::int32_t index_lo[3] = {dom.smallEnd(0), dom.smallEnd(1), dom.smallEnd(2)};
+            solver.init_expr(ops.data(), start.data(), index_lo);
+            return;
+        }
         const auto dx = geom.CellSizeArray();
         const auto lo = geom.ProbLoArray();
         const amrex::Box &dom = geom.Domain();
@@ -90,6 +106,7 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
 
     // nsteps calls of step(), with the device steps between exports replayed from a captured graph
     void advance(std::int64_t nsteps) {
+        ScopedTimer::StreamGuard on(solver.stream());
         while (nsteps > 0) {
             std::int64_t chunk = nsteps;
             if (exporter) {
@@ -100,7 +117,6 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
             {
                 ScopedTimer timer("Integrator::advance()");
                 device_steps(chunk, true);
-                if (ScopedTimer::enabled) solver.sync();
             }
             const amrex::Real dt = model.dt_hint();
             for (std::int64_t q = 0; q < chunk; ++q) {
@@ -129,11 +145,10 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
         if (exporter) exporter->exportIfNecessary((std::int64_t)steps, time);
     }
     amrex::Real step_impl() final override {
-        const amrex::Real dt = computeNewState(model.dt_hint());
-        if (ScopedTimer::enabled) solver.sync();
-        return dt;
+        return computeNewState(model.dt_hint());
     }
     void post_step() final override {}
+    void *timing_stream() const final override { return solver.stream(); }
 
     virtual amrex::Real computeNewState(amrex::Real dt_hint) = 0;
     virtual void device_steps(std::int64_t n, bool graph) = 0;

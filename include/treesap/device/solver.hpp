@@ -54,6 +54,12 @@ class Solver {
         if (l_) check(tsp_level_set_state(l_, host, 0), "tsp_level_set_state");
         else check(tsp_set_state(s_, host, 0), "tsp_set_state");
     }
+    // initialise_impl on the device: postfix expression programs (FunctionInit::program)
+    // evaluated at the cell centres; index_lo is the domain's first cell index
+    void init_expr(const tsp_expr_op *ops, const std::int32_t *start, const std::int32_t index_lo[3]) {
+        if (l_) check(tsp_level_init_expr(l_, ops, start, index_lo), "tsp_level_init_expr");
+        else check(tsp_init_expr(s_, ops, start, index_lo), "tsp_init_expr");
+    }
     void get_state(double *host) const {
         if (l_) check(tsp_level_get_state(l_, host, 0), "tsp_level_get_state");
         else check(tsp_get_state(s_, host, 0), "tsp_get_state");
@@ -112,6 +118,13 @@ class Solver {
         int32_t n = 0;
         check(tsp_batch_size(l_, &n, nullptr), "tsp_batch_size");
         return n;
+    }
+    // the hipStream_t the solve is enqueued on
+    void *stream() const {
+        void *st = nullptr;
+        if (l_) check(tsp_batch_get_stream(l_, &st), "tsp_batch_get_stream");
+        else if (s_) check(tsp_get_stream(s_, &st), "tsp_get_stream");
+        return st;
     }
     tsp_solver *handle() const { return s_; }   // single box (null for a level)
     tsp_batch *level() const { return l_; }     // a level (null for a single box)

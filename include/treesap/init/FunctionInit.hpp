@@ -14,17 +14,27 @@
 // abs fabs floor ceil pow min max heaviside. The functor itself only holds
 // pointers into a process-wide registry of parsed expressions, so it stays
 // trivially copyable as InitConcept requires.
+//
+// The integrators evaluate the initial data on the device: program() compiles
+// every component's expression into a postfix program (tsp_expr_op, codes of
+// treesap_hip.h; children in order, then the node), which tsp_init_expr runs
+// at every cell centre. + - * / sqrt floor ceil abs min max heaviside round as
+// on the host; the other functions use the device's libm, which may differ
+// from the host's in the last bits. operator()<c>(x, y, z) evaluates on the host.
 #pragma once
 
 #include "../pde-model/PDEModelConcepts.hpp"
+#include "treesap_hip.h"
 
 #include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <deque>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace treesap {
@@ -39,6 +49,8 @@ class Expression {
         if (pos_ != s_.size()) fail("unexpected '" + std::string(1, s_[pos_]) + "'");
     }
     double eval(double x, double y, double z) const { return eval(root_, x, y, z); }
+    // postfix program of the expression, appended to `out`
+    void emit(std::vector<tsp_expr_op> &out) const { emit(root_, out); }
 
   private:
     enum Op { NUM, VX, VY, VZ, ADD, SUB, MUL, DIV, NEG, POW, FN1, FN2 };
@@ -48,6 +60,7 @@ class Expression {
         int a = -1, b = -1;
         double (*f1)(double) = nullptr;
         double (*f2)(double, double) = nullptr;
+        int code = -1; // tsp_expr_code of a function node
     };
 
     int add(Node n) {
@@ -141,38 +154,41 @@ class Expression {
         struct F1 {
             const char *n;
             double (*f)(double);
+            int code;
         };
-        static const F1 one[] = {{"sqrt", [](double v) { return std::sqrt(v); }},
-                                 {"exp", [](double v) { return std::exp(v); }},
-                                 {"log", [](double v) { return std::log(v); }},
-                                 {"log10", [](double v) { return std::log10(v); }},
-                                 {"sin", [](double v) { return std::sin(v); }},
-                                 {"cos", [](double v) { return std::cos(v); }},
-                                 {"tan", [](double v) { return std::tan(v); }},
-                                 {"asin", [](double v) { return std::asin(v); }},
-                                 {"acos", [](double v) { return std::acos(v); }},
-                                 {"atan", [](double v) { return std::atan(v); }},
-                                 {"sinh", [](double v) { return std::sinh(v); }},
-                                 {"cosh", [](double v) { return std::cosh(v); }},
-                                 {"tanh", [](double v) { return std::tanh(v); }},
-                                 {"abs", [](double v) { return std::fabs(v); }},
-                                 {"fabs", [](double v) { return std::fabs(v); }},
-                                 {"floor", [](double v) { return std::floor(v); }},
-                                 {"ceil", [](double v) { return std::ceil(v); }},
-                                 {"heaviside", [](double v) { return v > 0.0 ? 1.0 : 0.0; }}};
+        static const F1 one[] = {{"sqrt", [](double v) { return std::sqrt(v); }, TSP_EXPR_SQRT},
+                                 {"exp", [](double v) { return std::exp(v); }, TSP_EXPR_EXP},
+                                 {"log", [](double v) { return std::log(v); }, TSP_EXPR_LOG},
+                                 {"log10", [](double v) { return std::log10(v); }, TSP_EXPR_LOG10},
+                                 {"sin", [](double v) { return std::sin(v); }, TSP_EXPR_SIN},
+                                 {"cos", [](double v) { return std::cos(v); }, TSP_EXPR_COS},
+                                 {"tan", [](double v) { return std::tan(v); }, TSP_EXPR_TAN},
+                                 {"asin", [](double v) { return std::asin(v); }, TSP_EXPR_ASIN},
+                                 {"acos", [](double v) { return std::acos(v); }, TSP_EXPR_ACOS},
+                                 {"atan", [](double v) { return std::atan(v); }, TSP_EXPR_ATAN},
+                                 {"sinh", [](double v) { return std::sinh(v); }, TSP_EXPR_SINH},
+                                 {"cosh", [](double v) { return std::cosh(v); }, TSP_EXPR_COSH},
+                                 {"tanh", [](double v) { return std::tanh(v); }, TSP_EXPR_TANH},
+                                 {"abs", [](double v) { return std::fabs(v); }, TSP_EXPR_ABS},
+                                 {"fabs", [](double v) { return std::fabs(v); }, TSP_EXPR_ABS},
+                                 {"floor", [](double v) { return std::floor(v); }, TSP_EXPR_FLOOR},
+                                 {"ceil", [](double v) { return std::ceil(v); }, TSP_EXPR_CEIL},
+                                 {"heaviside", [](double v) { return v > 0.0 ? 1.0 : 0.0; }, TSP_EXPR_HEAVISIDE}};
         struct F2 {
             const char *n;
             double (*f)(double, double);
+            int code;
         };
-        static const F2 two[] = {{"pow", [](double a, double b) { return std::pow(a, b); }},
-                                 {"atan2", [](double a, double b) { return std::atan2(a, b); }},
-                                 {"min", [](double a, double b) { return std::fmin(a, b); }},
-                                 {"max", [](double a, double b) { return std::fmax(a, b); }}};
+        static const F2 two[] = {{"pow", [](double a, double b) { return std::pow(a, b); }, TSP_EXPR_POW},
+                                 {"atan2", [](double a, double b) { return std::atan2(a, b); }, TSP_EXPR_ATAN2},
+                                 {"min", [](double a, double b) { return std::fmin(a, b); }, TSP_EXPR_MIN},
+                                 {"max", [](double a, double b) { return std::fmax(a, b); }, TSP_EXPR_MAX}};
         for (const auto &f : one)
             if (name == f.n) {
                 if (args.size() != 1) fail(name + " takes one argument");
                 Node n{FN1, 0.0, args[0]};
                 n.f1 = f.f;
+                n.code = f.code;
                 return add(n);
             }
         for (const auto &f : two)
@@ -180,6 +196,7 @@ class Expression {
                 if (args.size() != 2) fail(name + " takes two arguments");
                 Node n{FN2, 0.0, args[0], args[1]};
                 n.f2 = f.f;
+                n.code = f.code;
                 return add(n);
             }
         fail("unknown function '" + name + "'");
@@ -202,6 +219,44 @@ class Expression {
         case FN2: return n.f2(eval(n.a, x, y, z), eval(n.b, x, y, z));
         }
         return 0.0;
+    }
+
+    void emit(int i, std::vector<tsp_expr_op> &out) const {
+        const Node &n = nodes_[i];
+        static constexpr int leaf[] = {TSP_EXPR_NUM, TSP_EXPR_X, TSP_EXPR_Y, TSP_EXPR_Z};
+        static constexpr int binary[] = {TSP_EXPR_ADD, TSP_EXPR_SUB, TSP_EXPR_MUL, TSP_EXPR_DIV};
+        switch (n.op) {
+        case NUM:
+        case VX:
+        case VY:
+        case VZ: out.push_back({leaf[n.op - NUM], 0, n.op == NUM ? n.value : 0.0}); return;
+        case ADD:
+        case SUB:
+        case MUL:
+        case DIV:
+            emit(n.a, out);
+            emit(n.b, out);
+            out.push_back({binary[n.op - ADD], 0, 0.0});
+            return;
+        case NEG:
+            emit(n.a, out);
+            out.push_back({TSP_EXPR_NEG, 0, 0.0});
+            return;
+        case POW:
+            emit(n.a, out);
+            emit(n.b, out);
+            out.push_back({TSP_EXPR_POW, 0, 0.0});
+            return;
+        case FN1:
+            emit(n.a, out);
+            out.push_back({n.code, 0, 0.0});
+            return;
+        case FN2:
+            emit(n.a, out);
+            emit(n.b, out);
+            out.push_back({n.code, 0, 0.0});
+            return;
+        }
     }
 
     std::string s_;
@@ -235,10 +290,24 @@ template <int ncomp_> struct FunctionInit {
         return expr_[comp]->eval(x, y, z);
     }
 
+    // every component's postfix program, concatenated; component c is
+    // ops[start[c] .. start[c + 1]) (the layout tsp_init_expr reads)
+    void program(std::vector<tsp_expr_op> &ops, std::vector<std::int32_t> &start) const {
+        ops.clear();
+        start.assign(1, 0);
+        for (int c = 0; c < ncomp; ++c) {
+            expr_[c]->emit(ops);
+            start.push_back((std::int32_t)ops.size());
+        }
+    }
+
   private:
     const detail::Expression *expr_[ncomp]{};
 };
 
 static_assert(InitConcept<FunctionInit<1>>);
+
+template <class T> struct is_function_init : std::false_type {};
+template <int n> struct is_function_init<FunctionInit<n>> : std::true_type {};
 
 } // namespace treesap

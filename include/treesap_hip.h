@@ -143,6 +143,39 @@ tsp_status tsp_set_state(tsp_solver *s, const double *src, int32_t on_device);
 /* The reference has no state accessor (plotfiles only, PlotExporter.hpp:54-59). */
 tsp_status tsp_get_state(tsp_solver *s, double *dst, int32_t on_device);
 
+/* Initial data from expressions, evaluated on the device: replaces
+ * SingleLevelIntegrator::initialise_impl (SingleLevelIntegrator.hpp:75-93)
+ * with a FunctionInit (init/FunctionInit.hpp:34-101), whose amrex::Parser
+ * executors the reference calls on the device at every cell centre
+ * x = prob_lo + (i + 0.5) dx. Each component is a postfix program of
+ * tsp_expr_op (operands first, then the operation): NUM pushes `value`, X/Y/Z
+ * push the cell centre's coordinate, binary codes pop b then a and push
+ * a op b (POW: pow(a, b), ATAN2: atan2(a, b), MIN/MAX: fmin/fmax), unary
+ * codes replace the top value (HEAVISIDE: 1 if v > 0 else 0). */
+typedef enum tsp_expr_code {
+    TSP_EXPR_NUM = 0, TSP_EXPR_X = 1, TSP_EXPR_Y = 2, TSP_EXPR_Z = 3,
+    TSP_EXPR_ADD = 4, TSP_EXPR_SUB = 5, TSP_EXPR_MUL = 6, TSP_EXPR_DIV = 7, TSP_EXPR_POW = 8,
+    TSP_EXPR_ATAN2 = 9, TSP_EXPR_MIN = 10, TSP_EXPR_MAX = 11,
+    TSP_EXPR_NEG = 12, TSP_EXPR_SQRT = 13, TSP_EXPR_EXP = 14, TSP_EXPR_LOG = 15, TSP_EXPR_LOG10 = 16,
+    TSP_EXPR_SIN = 17, TSP_EXPR_COS = 18, TSP_EXPR_TAN = 19, TSP_EXPR_ASIN = 20, TSP_EXPR_ACOS = 21,
+    TSP_EXPR_ATAN = 22, TSP_EXPR_SINH = 23, TSP_EXPR_COSH = 24, TSP_EXPR_TANH = 25, TSP_EXPR_ABS = 26,
+    TSP_EXPR_FLOOR = 27, TSP_EXPR_CEIL = 28, TSP_EXPR_HEAVISIDE = 29
+} tsp_expr_code;
+typedef struct tsp_expr_op {
+    int32_t code;     /* tsp_expr_code */
+    int32_t reserved; /* 0 */
+    double value;     /* TSP_EXPR_NUM only */
+} tsp_expr_op;
+#define TSP_EXPR_MAX_STACK 16  /* values a program may hold at once */
+#define TSP_EXPR_MAX_OPS 65536 /* operations of all components together */
+/* Component c's program is ops[comp_start[c] .. comp_start[c+1]), comp_start[0] = 0.
+ * index_lo: the domain's first cell index (amrex::Box smallEnd; NULL: 0, 0, 0).
+ * The programs are checked first (TSP_INVALID_ARG: unknown code, underflow,
+ * not exactly one value left; TSP_UNSUPPORTED: deeper than TSP_EXPR_MAX_STACK).
+ * Writes state_old's valid cells and resets time, step count and register roles
+ * like tsp_set_state; returns when the values are written. */
+tsp_status tsp_init_expr(tsp_solver *s, const tsp_expr_op *ops, const int32_t *comp_start, const int32_t index_lo[3]);
+
 /* Replaces SingleLevelIntegrator::FillPatch (SingleLevelIntegrator.hpp:153-163)
  * on the register with role `role` (tsp_role): periodic images then the
  * physical-boundary functor, faces, edges and corners. */
@@ -209,6 +242,23 @@ tsp_status tsp_time_rk4(tsp_solver *s, int64_t nsteps, int32_t use_graph, double
 
 /* Block until the solver's stream is idle. */
 tsp_status tsp_sync(tsp_solver *s);
+
+/* The hipStream_t the solver enqueues its work on. */
+tsp_status tsp_get_stream(tsp_solver *s, void **stream);
+
+/* Device-side scope timer behind treesap::ScopedTimer
+ * (util/ScopedTimer.hpp:34-149, which reads the host clock): a pair of HIP
+ * events on one stream (NULL: the null stream). tsp_timer_start records the
+ * first; tsp_timer_stop records the second, waits for it and writes the
+ * device time between the two in ms, i.e. the time the stream spent on the
+ * work enqueued between the calls, without the host's launch overhead or any
+ * earlier work still queued when the scope began. A timer may be started
+ * again after it stopped. */
+typedef struct tsp_timer tsp_timer;
+tsp_status tsp_timer_create(void *stream, tsp_timer **out);
+tsp_status tsp_timer_start(tsp_timer *t);
+tsp_status tsp_timer_stop(tsp_timer *t, double *elapsed_ms);
+tsp_status tsp_timer_destroy(tsp_timer *t);
 
 /* Device pointer of the register with the given role and its layout: element
  * (i,j,k,c), ghosts included (-G <= i < n+G), is at
@@ -284,8 +334,13 @@ tsp_status tsp_level_boxes(tsp_batch *b, int32_t *lo_hi);
  * the clock (initialise()). */
 tsp_status tsp_level_set_state(tsp_batch *b, const double *src, int32_t on_device);
 tsp_status tsp_level_get_state(tsp_batch *b, double *dst, int32_t on_device);
+/* tsp_init_expr on every box of a level (cell centres in domain coordinates). */
+tsp_status tsp_level_init_expr(tsp_batch *b, const tsp_expr_op *ops, const int32_t *comp_start,
+                               const int32_t index_lo[3]);
 tsp_status tsp_batch_time_rk4(tsp_batch *b, int64_t nsteps, int32_t use_graph, double *ms);
 tsp_status tsp_batch_sync(tsp_batch *b);
+/* The hipStream_t the batch (or level) enqueues its work on. */
+tsp_status tsp_batch_get_stream(tsp_batch *b, void **stream);
 
 /* Discrete adjoint (reverse mode) of the RK4 stepping, no reference
  * counterpart. For the linear (phi, Pi) system and a loss

@@ -6,11 +6,13 @@
 //   final.bin  the state after the steps
 //   meta.txt   "<integrator> <steps> <time as %.17g> <boxes of the level>"
 // The tests replay desc + init.bin on the CPU oracle and compare final.bin.
+// The "timer" scenario (not run by default) prints device-timed scopes.
 //
 //   dropin_gpu <outdir> [scenario ...]
 #include "treesap.hpp"
 #include "user_rhs_traits.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
@@ -147,6 +149,36 @@ static void scenario_tensor(const fs::path &dir) {
     dump(dir, integ, u0, "rk4");
 }
 
+// ScopedTimer on the device: 120 graph-replayed steps are queued without
+// waiting, then a scope enqueues 12 more; the scope's time covers its own 12
+// steps, not the queue ahead of it. Prints "host_us <host time of both>"
+// and the timers' lines (no files).
+static void scenario_timer(const fs::path &) {
+    const amrex::Geometry geom = box_geom(96, 96, 96, 0.0, 1.0, 1);
+    using RHSType = KODecorator<ScalarWave<4, 4>, 4>;
+    using ModelType = PDEModel<RHSType, FunctionInit<2>>;
+    ModelType model({"phi", "Pi"}, geom, 0.25 / 96, RHSType{{}, 0.2},
+                    FunctionInit<2>({"exp(-((x-0.5)^2 + (y-0.5)^2 + (z-0.5)^2)/0.1^2)", "0.0"}));
+    SingleLevelRK4Integrator<ModelType> integ(std::move(model), {96, 96, 96});
+    integ.initialise();
+    integ.advance(4); // captures the replayed graphs before timing
+    device::Solver &sv = integ.device_solver();
+    sv.sync();
+    ScopedTimer::enabled = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    sv.graph_rk4_steps(120);
+    {
+        ScopedTimer t("scope of 12 steps");
+        sv.graph_rk4_steps(12);
+    }
+    const double host_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    for (int s = 0; s < 3; ++s) integ.step();
+    integ.advance(8);
+    ScopedTimer::printOverview();
+    ScopedTimer::enabled = false;
+    std::printf("host_us %.1f\n", host_us);
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <outdir> [scenario ...]\n", argv[0]);
@@ -166,6 +198,7 @@ int main(int argc, char **argv) {
             else if (w == "nonlinear_euler") scenario_nonlinear_euler(dir);
             else if (w == "tensor") scenario_tensor(dir);
             else if (w == "level_euler") scenario_level_euler(dir);
+            else if (w == "timer") scenario_timer(dir);
             else {
                 std::fprintf(stderr, "unknown scenario %s\n", w.c_str());
                 return 2;

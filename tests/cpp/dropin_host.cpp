@@ -372,6 +372,7 @@ int main(int argc, char **argv) {
     }
     if (mode == "expr" && argc == 3) {
         // one "x y z<TAB>expression" per line of the file; prints the value of each
+        // and, after a tab, its postfix device program as "code value" pairs
         std::ifstream fin(argv[2]);
         std::string line;
         while (std::getline(fin, line)) {
@@ -381,7 +382,12 @@ int main(int argc, char **argv) {
             if (std::sscanf(line.substr(0, tab).c_str(), "%lf %lf %lf", &x, &y, &z) != 3) return 2;
             try {
                 const FunctionInit<1> f({line.substr(tab + 1)});
-                std::printf("%.17g\n", f.operator()<0>(x, y, z));
+                std::vector<tsp_expr_op> ops;
+                std::vector<std::int32_t> start;
+                f.program(ops, start);
+                std::printf("%.17g\t", f.operator()<0>(x, y, z));
+                for (const auto &op : ops) std::printf(" %d %.17g", op.code, op.value);
+                std::printf("\n");
             } catch (const std::exception &e) {
                 std::printf("error %s\n", e.what());
             }

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import re
 import subprocess
 import sys
 import zlib
@@ -76,24 +77,118 @@ EXPR_CASES = [
 EXPR_ERRORS = ["x + (y", "w * 2", "foo(x)", "sin(x, y)", "pow(x)", "x +", "3 4"]
 
 
-def test_function_init_expressions_match_python(tmp_path):
-    rng = np.random.default_rng(7)
-    pts = rng.uniform(-0.9, 0.9, size=(6, 3))
-    lines, want = [], []
-    for text, fn in EXPR_CASES:
-        for x, y, z in pts.tolist():
-            lines.append(f"{x!r} {y!r} {z!r}\t{text}")
-            want.append(fn(x, y, z))
-    for text in EXPR_ERRORS:
-        lines.append(f"0 0 0\t{text}")
+# The device program codes (tsp_expr_code), restated from their documented
+# meaning in include/treesap_hip.h with Python's math: the check of what the
+# drop-in FunctionInit emits and of what tsp_init_expr computes.
+_UNARY = {"neg": lambda v: -v, "sqrt": math.sqrt, "exp": math.exp, "log": math.log, "log10": math.log10,
+          "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin, "acos": math.acos,
+          "atan": math.atan, "sinh": math.sinh, "cosh": math.cosh, "tanh": math.tanh, "abs": abs,
+          "floor": lambda v: float(math.floor(v)), "ceil": lambda v: float(math.ceil(v)),
+          "heaviside": lambda v: 1.0 if v > 0 else 0.0}
+_BINARY = {"add": lambda a, b: a + b, "sub": lambda a, b: a - b, "mul": lambda a, b: a * b,
+           "div": lambda a, b: a / b, "pow": math.pow, "atan2": math.atan2, "min": min, "max": max}
+_CODE_NAME = {v: k for k, v in _lib.EXPR_CODES.items()}
+# codes whose device results round as the host's (IEEE operations, exact functions)
+_EXACT_CODES = {_lib.EXPR_CODES[k] for k in ("num", "x", "y", "z", "add", "sub", "mul", "div", "neg", "sqrt",
+                                             "abs", "floor", "ceil", "heaviside", "min", "max")}
+
+
+def run_postfix(prog, x, y, z):
+    st = []
+    for code, value in prog:
+        name = _CODE_NAME[code]
+        if name == "num":
+            st.append(value)
+        elif name in ("x", "y", "z"):
+            st.append({"x": x, "y": y, "z": z}[name])
+        elif name in _BINARY:
+            b = st.pop()
+            st.append(_BINARY[name](st.pop(), b))
+        else:
+            st.append(_UNARY[name](st.pop()))
+    assert len(st) == 1
+    return st[0]
+
+
+def _parse_program(text):
+    f = text.split()
+    return [(int(f[i]), float(f[i + 1])) for i in range(0, len(f), 2)]
+
+
+def _expr_programs(tmp_path, texts, points):
+    lines = [f"{x!r} {y!r} {z!r}\t{t}" for t in texts for x, y, z in points]
     f = tmp_path / "exprs.txt"
     f.write_text("\n".join(lines) + "\n")
     out = _run([cpp_build.binary("dropin_host"), "expr", f]).stdout.splitlines()
     assert len(out) == len(lines)
-    got = [float(v) for v in out[:len(want)]]
-    for g, w, line in zip(got, want, lines):
-        assert g == w, (line, g, w)
-    assert all(v.startswith("error ") for v in out[len(want):]), out[len(want):]
+    return lines, out
+
+
+def test_function_init_expressions_match_python(tmp_path):
+    """The host evaluation of every expression, and the postfix program it
+    compiles to for the device (run by the documented meaning of its codes),
+    equal Python's evaluation of the same arithmetic bit for bit."""
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-0.9, 0.9, size=(6, 3)).tolist()
+    lines, out = _expr_programs(tmp_path, [t for t, _ in EXPR_CASES] + EXPR_ERRORS, pts)
+    nwant = len(EXPR_CASES) * len(pts)
+    want = [fn(x, y, z) for _, fn in EXPR_CASES for x, y, z in pts]
+    for o, w, line in zip(out[:nwant], want, lines):
+        value, prog = o.split("\t")
+        assert float(value) == w, (line, value, w)
+        x, y, z = (float(v) for v in line.split("\t")[0].split())
+        assert run_postfix(_parse_program(prog), x, y, z) == w, (line, prog)
+    assert all(v.startswith("error ") for v in out[nwant:]), out[nwant:]
+    # the programs put operands before operations: "x - y" is x y sub, not y x sub
+    prog = _parse_program(out[[t for t, _ in EXPR_CASES].index("x - y - z") * len(pts)].split("\t")[1])
+    assert [_CODE_NAME[c] for c, _ in prog] == ["x", "y", "sub", "z", "sub"]
+
+
+@pytest.mark.gpu
+def test_device_function_init_matches_python(tmp_path):
+    """tsp_init_expr runs the postfix programs of every expression at the cell
+    centres x = lo + (i + 0.5) dx of a ragged box whose first cell has index
+    (3, -2, 5) (SingleLevelIntegrator.hpp:75-93): bit for bit where the program
+    uses only IEEE operations and exact functions, within 1e-13 relative
+    (device libm) elsewhere; malformed programs are rejected before launch."""
+    from treesap_amd import Solver
+    from treesap_amd.workloads import small_case
+    texts = [t for t, _ in EXPR_CASES] + ["x", "1.5", "y*z - x/3", "sqrt(x*x + y*y + z*z)"]
+    _, out = _expr_programs(tmp_path, texts, [(0.1, 0.2, 0.3)])
+    progs = [_parse_program(o.split("\t")[1]) for o in out]
+    n, lo, hi, index_lo = (9, 7, 6), (-0.93, -0.81, -0.72), (0.87, 0.74, 0.79), (3, -2, 5)  # no centre at 0
+    dx = [(hi[a] - lo[a]) / n[a] for a in range(3)]
+    xs = [[lo[a] + (index_lo[a] + i + 0.5) * dx[a] for i in range(n[a])] for a in range(3)]
+    case, _ = small_case("phipi", n=n, stencil_n=1, ko_order=0, periodic=(1, 1, 1), seed=3)
+    case = case.replace(prob_lo=lo, prob_hi=hi)
+    s = Solver(case)
+    lib = _lib.lib()
+    lo3 = (C.c_int32 * 3)(*index_lo)
+    for t, prog in zip(texts, progs):
+        # component 0: the expression; component 1: its negation
+        ops_py = prog + prog + [(_lib.EXPR_CODES["neg"], 0.0)]
+        ops = (_lib.TspExprOp * len(ops_py))(*[_lib.TspExprOp(c, 0, v) for c, v in ops_py])
+        start = (C.c_int32 * 3)(0, len(prog), len(ops_py))
+        _lib.check(lib.tsp_init_expr(s.handle, ops, start, lo3), "tsp_init_expr")
+        got = s.get_state()
+        want = np.empty(case.shape)
+        for k, z in enumerate(xs[2]):
+            for j, y in enumerate(xs[1]):
+                want[0, k, j] = [run_postfix(prog, x, y, z) for x in xs[0]]
+        want[1] = -want[0]
+        if all(c in _EXACT_CODES for c, _ in prog):
+            assert np.array_equal(got, want, equal_nan=True), t
+        else:
+            np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14 * np.nanmax(np.abs(want)), err_msg=t)
+    assert s.time == 0.0 and s.steps == 0
+    bad = {"underflow": [(_lib.EXPR_CODES["add"], 0.0)],
+           "two values left": [(_lib.EXPR_CODES["x"], 0.0), (_lib.EXPR_CODES["y"], 0.0)],
+           "unknown code": [(99, 0.0)],
+           "too deep": [(_lib.EXPR_CODES["x"], 0.0)] * (_lib.TSP_EXPR_MAX_STACK + 1)}
+    for why, prog in bad.items():
+        ops = (_lib.TspExprOp * len(prog))(*[_lib.TspExprOp(c, 0, v) for c, v in prog])
+        start = (C.c_int32 * 3)(0, len(prog), len(prog) + 0)
+        assert lib.tsp_init_expr(s.handle, ops, start, lo3) != 0, why
 
 
 # decorated registered right-hand sides, same ids as eval_case() in dropin_host.cpp
@@ -217,11 +312,11 @@ INIT_EXPRS = {
 @pytest.mark.gpu
 @pytest.mark.parametrize("scenario", SCENARIOS)
 def test_dropin_initial_state_matches_cell_centres(gpu_runs, scenario):
-    """initialise() evaluates the InitConcept functor at the cell centres
+    """initialise() evaluates the FunctionInit expressions at the cell centres
     x = lo + (i + 0.5) * dx with dx = (hi - lo) / n (SingleLevelIntegrator.hpp:75-93,
-    Geometry::CellSize): the uploaded state equals Python's evaluation of the
-    same expressions there, bit for bit. A half-cell shift, a dx from n - 1 or
-    a swapped axis fails it."""
+    Geometry::CellSize), on the device: the state equals Python's evaluation of
+    the same expressions there to 1e-13 (device libm). A half-cell shift, a dx
+    from n - 1 or a swapped axis fails it."""
     d = gpu_runs / scenario
     case = case_from_desc(_read_desc(d / "desc.bin"))
     lo, hi, n, fns = INIT_EXPRS[scenario]
@@ -234,10 +329,14 @@ def test_dropin_initial_state_matches_cell_centres(gpu_runs, scenario):
         for k, z in enumerate(xs[2]):
             for j, y in enumerate(xs[1]):
                 want[c, k, j] = [fn(x, y, z) for x in xs[0]]
-    assert np.array_equal(init, want), f"max abs diff {np.max(np.abs(init - want))}"
-    # the check has teeth: a half-cell shift of x changes the state
+    # FunctionInit runs on the device: exp, sin and cos may differ from the host's libm in the last bits
+    np.testing.assert_allclose(init, want, rtol=1e-13, atol=1e-14 * np.max(np.abs(want)))
+    # the check has teeth: a half-cell shift of x changes the state (on a row through the middle,
+    # where the data are not below the tolerance)
     shifted = [lo[0] + i * dx[0] for i in range(n[0])]
-    assert not np.array_equal(init[0, 0, 0], [fns[0](x, xs[1][0], xs[2][0]) for x in shifted])
+    jm, km = n[1] // 2, n[2] // 2
+    assert not np.allclose(init[0, km, jm], [fns[0](x, xs[1][jm], xs[2][km]) for x in shifted],
+                           rtol=1e-13, atol=1e-14 * np.max(np.abs(want)))
 
 
 @pytest.mark.gpu
@@ -257,6 +356,43 @@ def test_dropin_plotfiles(gpu_runs):
     assert np.array_equal(np.frombuffer(data, dtype=np.float64).reshape(case.shape), o.get_state())
     hdr = (d / "plt00004" / "Header").read_text().splitlines()
     assert hdr[0] == "HyperCLaw-V1.1" and hdr[1] == "4" and hdr[2:6] == ["phi", "u_0", "u_1", "u_2"]
+
+
+_TIMER_LINE = re.compile(r"^\[Timer: (.+?)\] Run: ([0-9.]+) (µs|ms|s) \| Total: ([0-9.]+) (µs|ms|s) \| "
+                         r"Avg: ([0-9.]+) (µs|ms|s) \| StdDev: ([0-9.]+) (µs|ms|s) \((\d+) runs\)$")
+_OVERVIEW_LINE = re.compile(r"^\[Timer: (.{35,}?)\s*\] Executions: +(\d+) \| Total: +([0-9.]+) (µs|ms|s) \| "
+                            r"Avg: +([0-9.]+) (µs|ms|s) \| StdDev: +([0-9.]+) (µs|ms|s)$")
+_UNIT_US = {"µs": 1.0, "ms": 1e3, "s": 1e6}
+
+
+@pytest.mark.gpu
+def test_scoped_timer_times_the_device(tmp_path):
+    """ScopedTimer (util/ScopedTimer.hpp:34-149) prints the reference's lines, and its
+    time is the device stream's: a scope that enqueues 12 steps behind 120 queued ones
+    reports about 12/132 of the host time around both, where a host clock (which
+    would wait for the queue) reports all of it."""
+    res = _run([cpp_build.binary("dropin_gpu"), tmp_path, "timer"])
+    lines = res.stdout.splitlines()
+    host_us = float(next(l for l in lines if l.startswith("host_us")).split()[1])
+    runs = {}
+    for l in lines:
+        m = _TIMER_LINE.match(l)
+        if m:
+            runs.setdefault(m.group(1), []).append(float(m.group(2)) * _UNIT_US[m.group(3)])
+    scope = runs["scope of 12 steps"]
+    assert len(scope) == 1
+    frac = scope[0] / host_us
+    print(f"scope {scope[0]:.1f} us of host {host_us:.1f} us ({frac:.3f})", flush=True)
+    assert 0.02 < frac < 0.35, frac
+    assert len(runs["Integrator::step() [STEP]"]) == 3 and len(runs["Integrator::step() [PRE]"]) == 3
+    assert len(runs["Integrator::advance()"]) == 1
+    # one RK4 step's device time against the 12-step scope's
+    assert 0.3 < min(runs["Integrator::step() [STEP]"]) * 12 / scope[0] < 3.0
+    start = lines.index("--- --- --- ScopedTimer Profiling Overview --- --- ---")
+    over = [_OVERVIEW_LINE.match(l) for l in lines[start + 1:] if l.startswith("[Timer:")]
+    assert all(over) and len(over) == len(runs)
+    totals = [float(m.group(3)) * _UNIT_US[m.group(4)] for m in over]
+    assert totals == sorted(totals, reverse=True)
 
 
 # ---- functors the library does not carry: the generic kernel (tests/cpp/dropin_generic.cpp) ----

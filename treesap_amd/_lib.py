@@ -81,6 +81,18 @@ class TspBatchCfg(C.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+# tsp_expr_code: postfix programs of initial-data expressions (tsp_init_expr)
+EXPR_CODES = {name: i for i, name in enumerate(
+    ["num", "x", "y", "z", "add", "sub", "mul", "div", "pow", "atan2", "min", "max", "neg", "sqrt", "exp",
+     "log", "log10", "sin", "cos", "tan", "asin", "acos", "atan", "sinh", "cosh", "tanh", "abs", "floor",
+     "ceil", "heaviside"])}
+TSP_EXPR_MAX_STACK = 16
+
+
+class TspExprOp(C.Structure):
+    _fields_ = [("code", C.c_int32), ("reserved", C.c_int32), ("value", C.c_double)]
+
+
 class TspError(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
         super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {message}")
@@ -100,6 +112,7 @@ PROTOTYPES = {
     "tsp_destroy": (C.c_int, [_VP]),
     "tsp_set_state": (C.c_int, [_VP, _VP, C.c_int32]),
     "tsp_get_state": (C.c_int, [_VP, _VP, C.c_int32]),
+    "tsp_init_expr": (C.c_int, [_VP, _VP, _VP, _VP]),
     "tsp_fill_ghosts": (C.c_int, [_VP, C.c_int32]),
     "tsp_get_register": (C.c_int, [_VP, C.c_int32, _VP]),
     "tsp_rk4_steps": (C.c_int, [_VP, C.c_int64]),
@@ -114,6 +127,12 @@ PROTOTYPES = {
     "tsp_profile_rk4": (C.c_int, [_VP, C.c_int64, _DP, _DP]),
     "tsp_time_rk4": (C.c_int, [_VP, C.c_int64, C.c_int32, _DP]),
     "tsp_sync": (C.c_int, [_VP]),
+    "tsp_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "tsp_batch_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "tsp_timer_create": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "tsp_timer_start": (C.c_int, [_VP]),
+    "tsp_timer_stop": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "tsp_timer_destroy": (C.c_int, [_VP]),
     "tsp_register_view": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP), _I64P, _I64P, _I64P]),
     "tsp_rk4_bytes_per_cell": (C.c_int64, [C.POINTER(TspDesc)]),
     "tsp_batch_create": (C.c_int, [C.POINTER(TspDesc), C.c_int32, _VP, C.POINTER(_VP)]),
@@ -133,6 +152,7 @@ PROTOTYPES = {
     "tsp_level_boxes": (C.c_int, [_VP, C.POINTER(C.c_int32)]),
     "tsp_level_set_state": (C.c_int, [_VP, _VP, C.c_int32]),
     "tsp_level_get_state": (C.c_int, [_VP, _VP, C.c_int32]),
+    "tsp_level_init_expr": (C.c_int, [_VP, _VP, _VP, _VP]),
     "tsp_batch_time_rk4": (C.c_int, [_VP, C.c_int64, C.c_int32, _DP]),
     "tsp_batch_sync": (C.c_int, [_VP]),
     "tsp_adjoint_create": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, C.POINTER(_VP)]),

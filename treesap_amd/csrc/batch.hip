@@ -577,6 +577,12 @@ tsp_status tsp_batch_sync(tsp_batch *b) {
     return TSP_OK;
 }
 
+tsp_status tsp_batch_get_stream(tsp_batch *b, void **stream) {
+    if (!b || !stream) return fail(TSP_INVALID_ARG, "NULL argument");
+    *stream = (void *)b->stream;
+    return TSP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // autotuner of the batched kernels
 // ---------------------------------------------------------------------------
@@ -865,6 +871,24 @@ tsp_status tsp_level_set_state(tsp_batch *b, const double *src, int32_t on_devic
 
 tsp_status tsp_level_get_state(tsp_batch *b, double *dst, int32_t on_device) {
     return level_copy(b, dst, on_device, false);
+}
+
+tsp_status tsp_level_init_expr(tsp_batch *b, const tsp_expr_op *ops, const int32_t *comp_start,
+                               const int32_t index_lo[3]) {
+    if (!b || !b->level) return fail(TSP_INVALID_ARG, "not a level (tsp_level_create)");
+    const int nc = desc_ncomp(b->ldesc);
+    if (tsp_status st = check_expr_programs(ops, comp_start, nc); st != TSP_OK) return st;
+    std::memcpy(b->role, kIdentityRoles, sizeof(b->role));
+    b->steps = 0;
+    b->time.assign(b->count, 0.0);
+    std::vector<tsp::InitBox> boxes(b->count);
+    for (int i = 0; i < b->count; ++i) {
+        const Layout &L = b->L[i];
+        boxes[i] = tsp::InitBox{batch_origin(b, i, b->role[TSP_ROLE_OLD]), L.sj, L.sk, L.sc,
+                                {b->d[i].n[0], b->d[i].n[1], b->d[i].n[2]},
+                                {b->box_lo[3 * i], b->box_lo[3 * i + 1], b->box_lo[3 * i + 2]}};
+    }
+    return run_init_expr(boxes, b->ldesc, ops, comp_start, nc, index_lo, b->stream);
 }
 
 } // extern "C"

@@ -500,6 +500,19 @@ tsp_status tsp_get_state(tsp_solver *s, double *dst, int32_t on_device) {
     return TSP_OK;
 }
 
+tsp_status tsp_init_expr(tsp_solver *s, const tsp_expr_op *ops, const int32_t *comp_start, const int32_t index_lo[3]) {
+    if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
+    if (tsp_status st = check_expr_programs(ops, comp_start, s->nc); st != TSP_OK) return st;
+    for (int r = 0; r < 4; ++r) s->role[r] = r; // as tsp_set_state: the clock and the roles restart
+    tsp::InitBox box{reg_origin(s, s->role[TSP_ROLE_OLD]), s->sj, s->sk, s->sc, {s->d.n[0], s->d.n[1], s->d.n[2]},
+                     {0, 0, 0}};
+    if (tsp_status st = run_init_expr({box}, s->d, ops, comp_start, s->nc, index_lo, s->stream); st != TSP_OK)
+        return st;
+    s->time = 0.0;
+    s->steps = 0;
+    return TSP_OK;
+}
+
 tsp_status tsp_fill_ghosts(tsp_solver *s, int32_t role) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     if (role < 0 || role > 3) return fail(TSP_INVALID_ARG, "role must be 0..3");
@@ -710,6 +723,65 @@ tsp_status tsp_set_kernel(tsp_solver *s, const tsp_kernel_cfg *cfg) {
 tsp_status tsp_sync(tsp_solver *s) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     TSP_HIP(hipStreamSynchronize(s->stream));
+    return TSP_OK;
+}
+
+tsp_status tsp_get_stream(tsp_solver *s, void **stream) {
+    if (!s || !stream) return fail(TSP_INVALID_ARG, "NULL argument");
+    *stream = (void *)s->stream;
+    return TSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device scope timer (ScopedTimer.hpp:34-149 measures the host clock)
+// ---------------------------------------------------------------------------
+
+struct tsp_timer {
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool running = false;
+};
+
+tsp_status tsp_timer_create(void *stream, tsp_timer **out) {
+    if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    tsp_timer *t = new (std::nothrow) tsp_timer;
+    if (!t) return fail(TSP_OOM, "host allocation failed");
+    t->stream = (hipStream_t)stream;
+    hipError_t e = hipEventCreate(&t->start);
+    if (e == hipSuccess) e = hipEventCreate(&t->stop);
+    if (e != hipSuccess) {
+        tsp_timer_destroy(t);
+        return fail(TSP_HIP_ERROR, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+    *out = t;
+    return TSP_OK;
+}
+
+tsp_status tsp_timer_start(tsp_timer *t) {
+    if (!t) return fail(TSP_INVALID_ARG, "timer is NULL");
+    TSP_HIP(hipEventRecord(t->start, t->stream));
+    t->running = true;
+    return TSP_OK;
+}
+
+tsp_status tsp_timer_stop(tsp_timer *t, double *elapsed_ms) {
+    if (!t || !elapsed_ms) return fail(TSP_INVALID_ARG, "NULL argument");
+    if (!t->running) return fail(TSP_INVALID_ARG, "timer was not started");
+    t->running = false;
+    TSP_HIP(hipEventRecord(t->stop, t->stream));
+    TSP_HIP(hipEventSynchronize(t->stop));
+    float ms = 0.0f;
+    TSP_HIP(hipEventElapsedTime(&ms, t->start, t->stop));
+    *elapsed_ms = ms;
+    return TSP_OK;
+}
+
+tsp_status tsp_timer_destroy(tsp_timer *t) {
+    if (!t) return TSP_OK;
+    if (t->start) (void)hipEventDestroy(t->start);
+    if (t->stop) (void)hipEventDestroy(t->stop);
+    delete t;
     return TSP_OK;
 }
 

@@ -180,4 +180,24 @@ inline tsp_status validate(const tsp_desc *d) {
     return TSP_OK;
 }
 
+// initial data from postfix expression programs (init_expr.hip)
+tsp_status check_expr_programs(const tsp_expr_op *ops, const int32_t *comp_start, int nc);
+
+} // namespace tsp_host
+
+namespace tsp {
+// one box whose state_old valid cells an expression program fills
+struct InitBox {
+    double *origin; // element (0, 0, 0, 0) of the box's state_old register
+    long long sj, sk, sc;
+    int n[3];
+    int off[3]; // the box's first cell, domain index relative to the domain's first cell
+};
+} // namespace tsp
+
+namespace tsp_host {
+// evaluate the (checked) programs into the boxes on `stream` and wait; `domain`
+// gives prob_lo, prob_hi and the domain's cell counts
+tsp_status run_init_expr(const std::vector<tsp::InitBox> &boxes, const tsp_desc &domain, const tsp_expr_op *ops,
+                         const int32_t *comp_start, int nc, const int32_t index_lo[3], hipStream_t stream);
 } // namespace tsp_host
