@@ -67,8 +67,13 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     }
 
     void configureExporter(const fs::path &path, int export_interval) {
+        const amrex::Box &dom = model.get_geometry().Domain();
+        std::vector<amrex::Box> boxes;
+        for (const auto &b : solver.box_bounds()) // domain-relative lo, hi of every box of the level
+            boxes.emplace_back(amrex::IntVect(dom.smallEnd(0) + b[0], dom.smallEnd(1) + b[1], dom.smallEnd(2) + b[2]),
+                               amrex::IntVect(dom.smallEnd(0) + b[3], dom.smallEnd(1) + b[4], dom.smallEnd(2) + b[5]));
         exporter.emplace([this](std::vector<amrex::Real> &v) { v = get_state(); }, model.get_geometry(),
-                         model.get_comp_names(), path, export_interval);
+                         model.get_comp_names(), path, export_interval, std::move(boxes));
         interval_ = export_interval;
     }
 

@@ -7,10 +7,12 @@
 
 #include "treesap_hip.h"
 
+#include <array>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace treesap::device {
 
@@ -125,6 +127,20 @@ class Solver {
         if (l_) check(tsp_batch_get_stream(l_, &st), "tsp_batch_get_stream");
         else if (s_) check(tsp_get_stream(s_, &st), "tsp_get_stream");
         return st;
+    }
+    // every box's lo and hi (inclusive, relative to the domain's first cell), in level order
+    std::vector<std::array<int, 6>> box_bounds() const {
+        const int nb = boxes();
+        std::vector<std::array<int, 6>> out(nb);
+        if (!l_) {
+            out[0] = {0, 0, 0, desc_.n[0] - 1, desc_.n[1] - 1, desc_.n[2] - 1};
+            return out;
+        }
+        std::vector<int32_t> lh(6 * (std::size_t)nb);
+        check(tsp_level_boxes(l_, lh.data()), "tsp_level_boxes");
+        for (int b = 0; b < nb; ++b)
+            for (int q = 0; q < 6; ++q) out[b][q] = lh[6 * b + q];
+        return out;
     }
     tsp_solver *handle() const { return s_; }   // single box (null for a level)
     tsp_batch *level() const { return l_; }     // a level (null for a single box)

@@ -1,10 +1,15 @@
 // PlotExporter.hpp -- writes the state every `export_interval` steps as a
-// single-level, single-box plotfile named path + step (zero-padded to 5
-// digits), like util/PlotExporter.hpp:29-67 does through
-// amrex::WriteSingleLevelPlotfile. The layout follows AMReX's documented
-// plotfile format (HyperCLaw-V1.1 text header, a Level_0/Cell_H FabArray
-// header and one binary FAB file with a native-order IEEE double descriptor);
-// the state is copied from the device for each export.
+// single-level plotfile named path + step (zero-padded to 5 digits), like
+// util/PlotExporter.hpp:29-67 does through amrex::WriteSingleLevelPlotfile.
+// The layout follows AMReX's documented plotfile format: a HyperCLaw-V1.1 text
+// header (with the physical extent of every box of the level), a
+// Level_0/Cell_H FabArray header (the box array, one FabOnDisk entry per box
+// with its byte offset, per-box component minima and maxima) and one binary
+// file Level_0/Cell_D_00000 holding one FAB per box, each a text line with the
+// native-order IEEE double descriptor, the box and the component count,
+// followed by the box's values (x fastest, component slowest). The boxes are
+// those of the integrator's level (one box unless block_size splits the
+// domain); the state is copied from the device for each export.
 #pragma once
 
 #include "../amrex_types.hpp"
@@ -28,11 +33,22 @@ namespace fs = std::filesystem;
 class PlotExporter {
   public:
     // `fetch` fills a host buffer with the valid state, a[((c*nz+k)*ny+j)*nx+i].
+    // `boxes`: the level's boxes (empty: the domain as one box); together they must tile the domain.
     PlotExporter(std::function<void(std::vector<amrex::Real> &)> fetch, const amrex::Geometry &geom,
-                 amrex::Vector<std::string> comp_names, fs::path path, int export_interval)
+                 amrex::Vector<std::string> comp_names, fs::path path, int export_interval,
+                 std::vector<amrex::Box> boxes = {})
         : fetch_(std::move(fetch)), geom_(geom), names_(std::move(comp_names)), path_(std::move(path)),
-          interval_(export_interval) {
+          interval_(export_interval), boxes_(std::move(boxes)) {
         if (interval_ <= 0) throw std::invalid_argument("treesap: export interval must be positive");
+        if (boxes_.empty()) boxes_.push_back(geom_.Domain());
+        std::int64_t cells = 0;
+        for (const auto &b : boxes_) {
+            for (int d = 0; d < 3; ++d)
+                if (b.smallEnd(d) < geom_.Domain().smallEnd(d) || b.bigEnd(d) > geom_.Domain().bigEnd(d))
+                    throw std::invalid_argument("treesap: plotfile box outside the domain");
+            cells += b.numPts();
+        }
+        if (cells != geom_.Domain().numPts()) throw std::invalid_argument("treesap: plotfile boxes do not tile the domain");
     }
 
     void exportIfNecessary(std::int64_t step, amrex::Real time) const {
@@ -57,6 +73,7 @@ class PlotExporter {
         if ((std::int64_t)state.size() != cells * nc)
             throw std::invalid_argument("treesap: state size does not match the geometry and component names");
         fs::create_directories(fs::path(dir) / "Level_0");
+        const int nbox = (int)boxes_.size();
 
         std::ofstream h(fs::path(dir) / "Header");
         h << std::setprecision(17);
@@ -71,35 +88,63 @@ class PlotExporter {
         h << step << ' ' << '\n';
         for (int d = 0; d < 3; ++d) h << geom_.CellSize(d) << ' ';
         h << '\n' << geom_.Coord() << '\n' << 0 << '\n';
-        h << 0 << ' ' << 1 << ' ' << time << '\n' << step << '\n';
-        for (int d = 0; d < 3; ++d)
-            h << geom_.ProbLo(d) + dom.smallEnd(d) * geom_.CellSize(d) << ' '
-              << geom_.ProbLo(d) + (dom.bigEnd(d) + 1) * geom_.CellSize(d) << '\n';
+        h << 0 << ' ' << nbox << ' ' << time << '\n' << step << '\n';
+        for (const auto &b : boxes_)
+            for (int d = 0; d < 3; ++d)
+                h << geom_.ProbLo(d) + b.smallEnd(d) * geom_.CellSize(d) << ' '
+                  << geom_.ProbLo(d) + (b.bigEnd(d) + 1) * geom_.CellSize(d) << '\n';
         h << "Level_0/Cell\n";
         if (!h) throw std::runtime_error("treesap: cannot write " + dir + "/Header");
 
+        // FABs, one per box, and their offsets in Cell_D_00000
+        std::ofstream fab(fs::path(dir) / "Level_0" / "Cell_D_00000", std::ios::binary);
+        std::vector<std::int64_t> offset(nbox);
+        std::vector<std::vector<double>> lo(nbox, std::vector<double>(nc)), hi(nbox, std::vector<double>(nc));
+        std::vector<double> buf;
+        const std::int64_t nx = dom.length(0), ny = dom.length(1), nz = dom.length(2);
+        for (int ib = 0; ib < nbox; ++ib) {
+            const amrex::Box &b = boxes_[ib];
+            const std::int64_t bc = b.numPts();
+            buf.resize((std::size_t)(bc * nc));
+            std::size_t q = 0;
+            for (int c = 0; c < nc; ++c)
+                for (int k = b.smallEnd(2); k <= b.bigEnd(2); ++k)
+                    for (int j = b.smallEnd(1); j <= b.bigEnd(1); ++j) {
+                        const std::int64_t row =
+                            ((c * nz + (k - dom.smallEnd(2))) * ny + (j - dom.smallEnd(1))) * nx - dom.smallEnd(0);
+                        for (int i = b.smallEnd(0); i <= b.bigEnd(0); ++i) buf[q++] = state[(std::size_t)(row + i)];
+                    }
+            for (int c = 0; c < nc; ++c) {
+                const auto first = buf.begin() + c * bc;
+                lo[ib][c] = *std::min_element(first, first + bc);
+                hi[ib][c] = *std::max_element(first, first + bc);
+            }
+            offset[ib] = (std::int64_t)fab.tellp();
+            fab << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_text(b) << ' ' << nc << '\n';
+            fab.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)(buf.size() * sizeof(double)));
+        }
+        if (!fab) throw std::runtime_error("treesap: cannot write " + dir + "/Level_0/Cell_D_00000");
+
         std::ofstream ch(fs::path(dir) / "Level_0" / "Cell_H");
         ch << 1 << '\n' << 1 << '\n' << nc << '\n' << 0 << '\n';
-        ch << "(1 0\n" << box_text(dom) << "\n)\n";
-        ch << 1 << '\n' << "FabOnDisk: Cell_D_00000 0\n\n";
-        std::vector<double> lo(nc), hi(nc);
-        for (int c = 0; c < nc; ++c) {
-            const auto b = state.begin() + c * cells;
-            lo[c] = cells ? *std::min_element(b, b + cells) : 0.0;
-            hi[c] = cells ? *std::max_element(b, b + cells) : 0.0;
-        }
-        ch << std::setprecision(16) << std::scientific;
-        ch << 1 << ',' << nc << '\n';
-        for (double v : lo) ch << v << ',';
-        ch << "\n\n" << 1 << ',' << nc << '\n';
-        for (double v : hi) ch << v << ',';
+        ch << '(' << nbox << " 0\n";
+        for (const auto &b : boxes_) ch << box_text(b) << '\n';
+        ch << ")\n";
+        ch << nbox << '\n';
+        for (int ib = 0; ib < nbox; ++ib) ch << "FabOnDisk: Cell_D_00000 " << offset[ib] << '\n';
         ch << '\n';
+        ch << std::setprecision(16) << std::scientific;
+        ch << nbox << ',' << nc << '\n';
+        for (int ib = 0; ib < nbox; ++ib) {
+            for (double v : lo[ib]) ch << v << ',';
+            ch << '\n';
+        }
+        ch << '\n' << nbox << ',' << nc << '\n';
+        for (int ib = 0; ib < nbox; ++ib) {
+            for (double v : hi[ib]) ch << v << ',';
+            ch << '\n';
+        }
         if (!ch) throw std::runtime_error("treesap: cannot write " + dir + "/Level_0/Cell_H");
-
-        std::ofstream fab(fs::path(dir) / "Level_0" / "Cell_D_00000", std::ios::binary);
-        fab << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_text(dom) << ' ' << nc << '\n';
-        fab.write(reinterpret_cast<const char *>(state.data()), (std::streamsize)(state.size() * sizeof(double)));
-        if (!fab) throw std::runtime_error("treesap: cannot write " + dir + "/Level_0/Cell_D_00000");
     }
 
   private:
@@ -115,6 +160,7 @@ class PlotExporter {
     amrex::Vector<std::string> names_;
     fs::path path_;
     int interval_;
+    std::vector<amrex::Box> boxes_;
 };
 
 } // namespace treesap

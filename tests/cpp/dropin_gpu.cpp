@@ -93,7 +93,8 @@ static void scenario_abl_ko(const fs::path &dir) {
 }
 
 // (phi, Pi) wave with KO on a periodic ragged box split into boxes of at most
-// 7 x 9 x 64 cells (a level of 3 x 2 x 1 boxes), forward Euler then RK4 steps
+// 7 x 9 x 64 cells (a level of 3 x 2 x 1 boxes), forward Euler steps, plotfiles
+// every 5 steps
 static void scenario_level_euler(const fs::path &dir) {
     const int p[3] = {1, 1, 1};
     const amrex::RealBox rb({0.0, 0.0, 0.0}, {1.0, 0.9, 0.8});
@@ -106,6 +107,7 @@ static void scenario_level_euler(const fs::path &dir) {
     SingleLevelEulerIntegrator<ModelType> integ(std::move(model), {7, 9, 64});
     integ.initialise();
     const auto u0 = integ.get_state();
+    integ.configureExporter(dir / "plt", 5);
     for (int s = 0; s < 6; ++s) integ.step();
     dump(dir, integ, u0, "euler");
 }

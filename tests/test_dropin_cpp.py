@@ -339,23 +339,99 @@ def test_dropin_initial_state_matches_cell_centres(gpu_runs, scenario):
                            rtol=1e-13, atol=1e-14 * np.max(np.abs(want)))
 
 
+_FAB_HEAD = re.compile(rb"^FAB \(\(8, \(64 11 52 0 1 12 0 1023\)\),\(8, \(8 7 6 5 4 3 2 1\)\)\)"
+                       rb"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(0,0,0\)\) (\d+)\n")
+_BOX_TEXT = re.compile(r"^\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(0,0,0\)\)$")
+
+
+def read_plotfile(path: Path):
+    """Reads a single-level plotfile by AMReX's documented layout (HyperCLaw-V1.1
+    Header, Level_0/Cell_H FabArray header, FABs at the offsets Cell_H gives) and
+    checks its internal consistency; returns (names, time, step, boxes, dense state
+    a[c, k, j, i] over the domain)."""
+    hdr = (path / "Header").read_text().split("\n")
+    assert hdr[0] == "HyperCLaw-V1.1"
+    nc = int(hdr[1])
+    names = hdr[2:2 + nc]
+    q = 2 + nc
+    assert hdr[q] == "3" and hdr[q + 2] == "0"  # dimensions, finest level
+    time = float(hdr[q + 1])
+    prob_lo = [float(v) for v in hdr[q + 3].split()]
+    prob_hi = [float(v) for v in hdr[q + 4].split()]
+    assert hdr[q + 5] == ""  # no refinement ratios on one level
+    dom = [int(v) for v in _BOX_TEXT.match(hdr[q + 6].strip()).groups()]
+    step = int(hdr[q + 7].split()[0])
+    dx = [float(v) for v in hdr[q + 8].split()]
+    assert hdr[q + 9] == "0" and hdr[q + 10] == "0"  # Cartesian, no boundary data
+    lev, nbox, ltime = hdr[q + 11].split()
+    assert lev == "0" and float(ltime) == time and int(hdr[q + 12]) == step
+    nbox = int(nbox)
+    extents = [[float(v) for v in hdr[q + 13 + 3 * b + d].split()] for b in range(nbox) for d in range(3)]
+    assert hdr[q + 13 + 3 * nbox] == "Level_0/Cell"
+    n = [dom[3 + d] - dom[d] + 1 for d in range(3)]
+    ch = (path / "Level_0" / "Cell_H").read_text().split("\n")
+    assert ch[0:4] == ["1", "1", str(nc), "0"]  # version, NFiles, components, no ghosts
+    assert ch[4] == f"({nbox} 0"
+    boxes = [[int(v) for v in _BOX_TEXT.match(ch[5 + b]).groups()] for b in range(nbox)]
+    assert ch[5 + nbox] == ")" and int(ch[6 + nbox]) == nbox
+    fod = [ch[7 + nbox + b].split() for b in range(nbox)]
+    assert all(f[0] == "FabOnDisk:" and f[1] == "Cell_D_00000" for f in fod) and ch[7 + 2 * nbox] == ""
+    r = 8 + 2 * nbox
+    assert ch[r] == f"{nbox},{nc}"
+    mins = [[float(v) for v in ch[r + 1 + b].rstrip(",").split(",")] for b in range(nbox)]
+    assert ch[r + 1 + nbox] == "" and ch[r + 2 + nbox] == f"{nbox},{nc}"
+    maxs = [[float(v) for v in ch[r + 3 + nbox + b].rstrip(",").split(",")] for b in range(nbox)]
+    raw = (path / "Level_0" / "Cell_D_00000").read_bytes()
+    dense = np.full((nc, n[2], n[1], n[0]), np.nan)
+    covered = np.zeros((n[2], n[1], n[0]), dtype=int)
+    pos = 0
+    for b, box in enumerate(boxes):
+        off = int(fod[b][2])
+        assert off == pos, "FABs follow each other in box order"
+        m = _FAB_HEAD.match(raw[off:off + 200])
+        assert m and [int(v) for v in m.groups()[:6]] == box and int(m.group(7)) == nc
+        bn = [box[3 + d] - box[d] + 1 for d in range(3)]
+        cnt = nc * bn[0] * bn[1] * bn[2]
+        data = np.frombuffer(raw, dtype="<f8", count=cnt, offset=off + m.end()).reshape(nc, bn[2], bn[1], bn[0])
+        pos = off + m.end() + 8 * cnt
+        sl = tuple(slice(box[d] - dom[d], box[3 + d] - dom[d] + 1) for d in (2, 1, 0))
+        dense[(slice(None),) + sl] = data
+        covered[sl] += 1
+        assert [float(v) for v in data.reshape(nc, -1).min(axis=1)] == mins[b]
+        assert [float(v) for v in data.reshape(nc, -1).max(axis=1)] == maxs[b]
+        for d in range(3):  # physical extent of the box in the Header
+            assert extents[3 * b + d] == [prob_lo[d] + box[d] * dx[d], prob_lo[d] + (box[3 + d] + 1) * dx[d]]
+    assert pos == len(raw) and np.all(covered == 1), "the boxes tile the domain exactly once"
+    assert [dx[d] == (prob_hi[d] - prob_lo[d]) / n[d] for d in range(3)] == [True] * 3
+    return names, time, step, boxes, dense
+
+
 @pytest.mark.gpu
 def test_dropin_plotfiles(gpu_runs):
-    """configureExporter(path, 4) writes the state before steps 0, 4 and 8 of 12."""
-    d = gpu_runs / "first_order"
-    case = case_from_desc(_read_desc(d / "desc.bin"))
-    init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
-    names = sorted(p.name for p in d.iterdir() if p.name.startswith("plt"))
-    assert names == ["plt00000", "plt00004", "plt00008"]
-    o = orc.OracleSolver(case)
-    o.set_state(init)
-    o.rk4_steps(4)
-    fab = (d / "plt00004" / "Level_0" / "Cell_D_00000").read_bytes()
-    header, data = fab.split(b"\n", 1)
-    assert header.startswith(b"FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))((0,0,0) (31,31,31)")
-    assert np.array_equal(np.frombuffer(data, dtype=np.float64).reshape(case.shape), o.get_state())
-    hdr = (d / "plt00004" / "Header").read_text().splitlines()
-    assert hdr[0] == "HyperCLaw-V1.1" and hdr[1] == "4" and hdr[2:6] == ["phi", "u_0", "u_1", "u_2"]
+    """configureExporter(path, k) writes the state before every k-th step: on one
+    box (the first-order wave, exports at steps 0, 4, 8 of 12) and on a level of
+    3 x 2 x 1 boxes (block size 7 x 9 x 64 on 20 x 18 x 16 cells, exports at 0 and
+    5 of 6), one FAB per box. Every file is read back by the documented layout and
+    its payload equals the oracle's state at that step bit for bit."""
+    for scenario, names_want, every, steps, kind in (("first_order", ["phi", "u_0", "u_1", "u_2"], 4, 12, "rk4"),
+                                                     ("level_euler", ["phi", "Pi"], 5, 6, "euler")):
+        d = gpu_runs / scenario
+        case = case_from_desc(_read_desc(d / "desc.bin"))
+        init = np.fromfile(d / "init.bin", dtype=np.float64).reshape(case.shape)
+        names = sorted(p.name for p in d.iterdir() if p.name.startswith("plt"))
+        assert names == [f"plt{s:05d}" for s in range(0, steps, every)], names
+        o = orc.OracleSolver(case)
+        o.set_state(init)
+        for s in range(0, steps, every):
+            if s:
+                o.rk4_steps(every) if kind == "rk4" else o.euler_steps(every)
+            got_names, t, step, boxes, dense = read_plotfile(d / f"plt{s:05d}")
+            assert got_names == names_want and step == s and t == o.time
+            assert np.array_equal(dense, o.get_state()), (scenario, s)
+        if scenario == "level_euler":
+            assert len(boxes) == 6
+            assert sorted({(b[0], b[3]) for b in boxes}) == [(0, 6), (7, 13), (14, 19)]
+            assert sorted({(b[1], b[4]) for b in boxes}) == [(0, 8), (9, 17)]
 
 
 _TIMER_LINE = re.compile(r"^\[Timer: (.+?)\] Run: ([0-9.]+) (µs|ms|s) \| Total: ([0-9.]+) (µs|ms|s) \| "
