@@ -9,9 +9,10 @@
 // As in the reference, the level is split into boxes of at most block_size
 // cells per axis (SingleLevelIntegrator.hpp:47-49,172-176), each box with its
 // own registers, and FillPatch exchanges ghosts between the boxes before every
-// stage (tsp_level_*); results do not depend on the split. The (phi, Pi)
-// system runs split levels; the other systems run the level as one box, which
-// gives the same results.
+// stage (tsp_level_*); results do not depend on the split. The (phi, Pi),
+// (phi, u) and NonLinear systems run split levels; the tensor system and
+// functors on the generic kernel run the level as one box, which gives the
+// same results.
 // initialise() evaluates the initial condition at cell centres
 // x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93): a FunctionInit
 // on the device (its expressions compiled to postfix programs), any other

@@ -265,10 +265,11 @@ tsp_status tsp_timer_destroy(tsp_timer *t);
  *   ptr[i + j*sj + k*sk + c*sc]   with ptr pointing at (0,0,0,0). */
 tsp_status tsp_register_view(tsp_solver *s, int32_t role, double **ptr, int64_t *sj, int64_t *sk, int64_t *sc);
 
-/* Batched mode (no reference counterpart): many independent (phi, Pi)
- * models of uneven sizes, stencil widths and KO orders advanced in lockstep.
- * Per RK4 stage there is one persistent launch per (stencil width, KO order)
- * group present in the batch, covering every instance of the group; steps can
+/* Batched mode (no reference counterpart): many independent models of the
+ * (phi, Pi), (phi, u) and NonLinear systems, of uneven sizes, stencil widths
+ * and KO orders, advanced in lockstep. Per RK4 stage there is one persistent
+ * launch per (system, stencil width, KO order) group present in the batch,
+ * covering every instance of the group; steps can
  * be replayed from a captured hipGraph. Each instance's result is
  * bit-identical to solving it alone with tsp_create/tsp_rk4_steps (the same
  * tile body runs). */
@@ -326,7 +327,8 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
  * and corners) and the Dirichlet values are written outside non-periodic
  * faces; the stage kernels then read ghost memory. The absorbing layer is
  * measured in domain coordinates. Results are bit-identical to a single box
- * of the whole domain for every block size. (phi, Pi) system only. */
+ * of the whole domain for every block size. (phi, Pi), (phi, u) and NonLinear
+ * systems. */
 tsp_status tsp_level_create(const tsp_desc *d, const int32_t block[3], void *stream, tsp_batch **out);
 /* Box b's index range, lo_hi[6*b + 0..2] = lo (x, y, z), [6*b + 3..5] = hi (inclusive). */
 tsp_status tsp_level_boxes(tsp_batch *b, int32_t *lo_hi);

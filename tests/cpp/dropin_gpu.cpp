@@ -53,7 +53,8 @@ static amrex::Geometry box_geom(int nx, int ny, int nz, double lo, double hi, in
     return amrex::Geometry(amrex::Box({0, 0, 0}, {nx - 1, ny - 1, nz - 1}), &rb, amrex::CoordSys::cartesian, p);
 }
 
-// (phi, u) wave, Dirichlet zero on a non-periodic [-2, 2]^3, plotfiles every 4 steps
+// (phi, u) wave, Dirichlet zero on a non-periodic [-2, 2]^3 split into 2 x 2 x 2
+// boxes, plotfiles every 4 steps
 static void scenario_first_order(const fs::path &dir) {
     const amrex::Geometry geom = box_geom(32, 32, 32, -2.0, 2.0, 0);
     constexpr int ngrow = 1;
@@ -112,7 +113,8 @@ static void scenario_level_euler(const fs::path &dir) {
     dump(dir, integ, u0, "euler");
 }
 
-// nonlinear wave with a workspace temporary, KO, periodic ragged box, forward Euler
+// nonlinear wave with a workspace temporary, KO, periodic ragged box split into
+// boxes of at most 16 x 20 x 64 cells, forward Euler
 static void scenario_nonlinear_euler(const fs::path &dir) {
     const int p[3] = {1, 1, 1};
     const amrex::RealBox rb({0.0, 0.0, 0.0}, {1.0, 0.9, 0.8});
@@ -122,7 +124,7 @@ static void scenario_nonlinear_euler(const fs::path &dir) {
     using ModelType = PDEModel<RHSType, InitType>;
     InitType init({"sin(2*pi*x)*cos(2*pi*y/0.9) + 0.3*sin(2*pi*z/0.8)", "0.5*cos(2*pi*(x + y))"});
     ModelType model({"phi", "pi"}, geom, 0.004, RHSType{{0.05}, 0.1}, init);
-    SingleLevelEulerIntegrator<ModelType> integ(std::move(model), {64, 64, 64});
+    SingleLevelEulerIntegrator<ModelType> integ(std::move(model), {16, 20, 64}); // a level of 3 x 2 x 1 boxes
     integ.initialise();
     const auto u0 = integ.get_state();
     for (int s = 0; s < 8; ++s) integ.step();

@@ -120,7 +120,7 @@ def test_batch_validation():
 
     good = Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01)
     with pytest.raises(_lib.TspError, match="batched mode runs the"):
-        Batch([good, good.replace(system="phiu")])
+        Batch([good, good.replace(system="tensor", stencil_n=2, ghost=2, ko_order=2)])
     with pytest.raises(_lib.TspError, match="no batched kernel"):
         Batch([good.replace(stencil_n=3, ghost=4, ko_order=4)])
     with pytest.raises(_lib.TspError, match="instance 1: .*stencil_n"):
@@ -163,12 +163,12 @@ def test_custom_model_validation():
 
 
 def test_level_validation():
-    """tsp_level_create rejects bad block sizes and systems other than (phi, Pi) before any HIP call."""
+    """tsp_level_create rejects bad block sizes and the tensor system before any HIP call."""
     L = _lib.lib()
     h = C.c_void_p()
     d = make_desc(Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01))
     assert L.tsp_level_create(C.byref(d), (C.c_int32 * 3)(8, 0, 8), None, C.byref(h)) == _lib.INVALID_ARG
     assert L.tsp_level_create(C.byref(d), None, None, C.byref(h)) == _lib.INVALID_ARG
-    d2 = make_desc(Case(system="phiu", n=(16, 16, 16), stencil_n=2, ghost=2, dt=0.01))
+    d2 = make_desc(Case(system="tensor", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01))
     assert L.tsp_level_create(C.byref(d2), (C.c_int32 * 3)(8, 8, 8), None, C.byref(h)) == _lib.UNSUPPORTED
-    assert "(phi,Pi)" in L.tsp_last_error().decode()
+    assert "(phi,Pi), (phi,u) and NonLinear" in L.tsp_last_error().decode()

@@ -241,8 +241,9 @@ def test_registered_rhs_matches_oracle(tmp_path, case_id, geometry):
 # ---- whole programs on the GPU ------------------------------------------------------
 
 SCENARIOS = ["first_order", "abl_ko", "nonlinear_euler", "tensor", "level_euler"]
-# boxes of the level each program's block size gives: (phi, Pi) models are split
-LEVEL_BOXES = {"first_order": 1, "abl_ko": 8, "nonlinear_euler": 1, "tensor": 1, "level_euler": 6}
+# boxes of the level each program's block size gives ((phi, Pi), (phi, u) and NonLinear models are
+# split; the tensor system runs one box)
+LEVEL_BOXES = {"first_order": 8, "abl_ko": 8, "nonlinear_euler": 6, "tensor": 1, "level_euler": 6}
 
 
 def _read_desc(path: Path) -> _lib.TspDesc:
@@ -408,10 +409,10 @@ def read_plotfile(path: Path):
 
 @pytest.mark.gpu
 def test_dropin_plotfiles(gpu_runs):
-    """configureExporter(path, k) writes the state before every k-th step: on one
-    box (the first-order wave, exports at steps 0, 4, 8 of 12) and on a level of
-    3 x 2 x 1 boxes (block size 7 x 9 x 64 on 20 x 18 x 16 cells, exports at 0 and
-    5 of 6), one FAB per box. Every file is read back by the documented layout and
+    """configureExporter(path, k) writes the state before every k-th step: on a
+    level of 2 x 2 x 2 boxes (the first-order wave, exports at steps 0, 4, 8 of 12)
+    and on one of 3 x 2 x 1 boxes (block size 7 x 9 x 64 on 20 x 18 x 16 cells,
+    exports at 0 and 5 of 6), one FAB per box. Every file is read back by the documented layout and
     its payload equals the oracle's state at that step bit for bit."""
     for scenario, names_want, every, steps, kind in (("first_order", ["phi", "u_0", "u_1", "u_2"], 4, 12, "rk4"),
                                                      ("level_euler", ["phi", "Pi"], 5, 6, "euler")):

@@ -182,3 +182,42 @@ def test_batch_autotune_restores_state_and_keeps_results():
         o.set_state(st)
         o.rk4_steps(3)
         assert np.array_equal(b.get_state(i), o.get_state()), i
+
+
+def test_mixed_system_batch_matches_oracle():
+    """(phi, Pi), (phi, u) and NonLinear instances in one batch: one launch per stage and
+    (system, stencil width, KO order) group; every instance bitwise equal to its single-box
+    solve and to the oracle, eager and graph-replayed, RK4 and Euler, under two batch
+    configurations."""
+    _say("[batch] mixed systems")
+    specs = [("phiu", (40, 22, 18), 2, 2, (0, 0, 0), 4, 0.0), ("nonlinear", (33, 20, 16), 2, 3, (1, 1, 1), 0, 0.3),
+             ("phipi", (24, 20, 16), 3, 3, (1, 0, 1), 0, 0.0), ("nonlinear", (70, 12, 14), 4, 4, (0, 0, 0), 3, 0.1),
+             ("phiu", (20, 18, 24), 1, 1, (1, 1, 1), 0, 0.0), ("phiu", (64, 16, 10), 4, 0, (0, 1, 0), 0, 0.0),
+             ("nonlinear", (17, 9, 12), 1, 0, (0, 1, 0), 0, 0.2)]
+    cases = []
+    for k, (sysname, n, sn, ko, per, abl, gamma) in enumerate(specs):
+        case, state = small_case(sysname, n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=70 + k,
+                                 gamma=gamma)
+        cases.append((case.replace(dt=case.dt * (1.0 + 0.07 * k)), state))
+    for cfg, graph in ((BATCH_CFGS[0], False), (BATCH_CFGS[1], True)):
+        b = Batch([c for c, _ in cases])
+        b.kernel = cfg
+        for i, (_, st) in enumerate(cases):
+            b.set_state(i, st)
+        if graph:
+            b.graph_rk4_steps(4)
+        else:
+            b.rk4_steps(4)
+        b.euler_steps(1)
+        for i, (case, st) in enumerate(cases):
+            o = orc.OracleSolver(case)
+            o.set_state(st)
+            o.rk4_steps(4)
+            o.euler_steps(1)
+            assert np.array_equal(b.get_state(i), o.get_state()), (i, case.system, cfg)
+            s = Solver(case)
+            s.set_state(st)
+            s.rk4_steps(4)
+            s.euler_steps(1)
+            assert np.array_equal(b.get_state(i), s.get_state()), (i, case.system)
+        b.close()

@@ -80,3 +80,25 @@ def test_level_euler_and_kernel_configs():
         lv.rk4_steps(3)
         assert np.array_equal(lv.get_state(), want_r)
         lv.close()
+
+
+@pytest.mark.parametrize("system,gamma", [("phiu", 0.0), ("nonlinear", 0.3)])
+def test_level_of_other_systems(system, gamma):
+    """(phi, u) and NonLinear levels: every block size bitwise equal to the oracle's single box
+    (Dirichlet faces with an absorbing layer, and a mixed-periodic box)."""
+    for n, sn, ko, per, abl, blocks in (((30, 26, 22), 2, 2, (0, 0, 0), 4, [(11, 9, 22), (6, 6, 6)]),
+                                        ((24, 20, 18), 3, 3, (1, 0, 1), 0, [(10, 7, 18), (24, 20, 5)])):
+        case, state = small_case(system, n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=33, gamma=gamma)
+        o = orc.OracleSolver(case)
+        o.set_state(state)
+        o.rk4_steps(4)
+        want = o.get_state()
+        for blk in blocks:
+            _say(f"[level] {system} n={n} block={blk}")
+            lv = Level(case, blk)
+            lv.set_state(state)
+            lv.rk4_steps(1)
+            lv.graph_rk4_steps(3)
+            got = lv.get_state()
+            assert np.array_equal(got, want), f"{system} block {blk}: max abs diff {np.max(np.abs(got - want))}"
+            lv.close()

@@ -1,5 +1,5 @@
 // batch.hip -- the batched half of the C ABI (tsp_batch_*) and the batched
-// stage kernels of the (phi, Pi) system.
+// stage kernels of the (phi, Pi) system (generic body).
 //
 // A batch is many independent single-box models advanced together. All
 // registers of all instances live in one pool allocation (each register keeps
@@ -27,6 +27,16 @@ hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int
     if (b.stage == kStage3) return launch_batched_phipi_s3(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
     return launch_batched_phipi_mode<-1>(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
 }
+hipError_t launch_batched_group(int group, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                                hipStream_t stream) {
+    const int v = group % kBatchVariants;
+    switch (group / kBatchVariants) {
+    case 0: return launch_batched_phipi(v, b, tile_y, grid, max_abl_T, halo_ring, stream);
+    case 1: return launch_batched_phiu(v, b, tile_y, grid, max_abl_T, halo_ring, stream);
+    case 2: return launch_batched_nonlinear(v, b, tile_y, grid, max_abl_T, halo_ring, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
 } // namespace tsp
 
 using namespace tsp_host;
@@ -44,8 +54,8 @@ struct tsp_batch {
     tsp::BatchInstance *d_inst = nullptr;
     tsp::BatchWork *d_work = nullptr;
     int nwork = 0;
-    int group_off[tsp::kBatchVariants] = {}; // work items of variant v: [group_off[v], group_off[v] + group_n[v])
-    int group_n[tsp::kBatchVariants] = {};
+    int group_off[tsp::kBatchGroups] = {}; // work items of group v: [group_off[v], group_off[v] + group_n[v])
+    int group_n[tsp::kBatchGroups] = {};
     int role[4] = {0, 1, 2, 3};
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -122,7 +132,7 @@ tsp_status build_work(tsp_batch *b) {
     });
     std::vector<tsp::BatchWork> work;
     const int ty = b->cfg.tile_y;
-    for (int v = 0; v < tsp::kBatchVariants; ++v) {
+    for (int v = 0; v < tsp::kBatchGroups; ++v) {
         b->group_off[v] = (int)work.size();
         for (int i : order) {
             if (b->h_inst[i].variant != v) continue;
@@ -159,7 +169,7 @@ tsp_status build_work(tsp_batch *b) {
 }
 
 // One RK4 step of every instance: per stage, one persistent launch per
-// (stencil width, KO order) group present in the batch.
+// (system, stencil width, KO order) group present in the batch.
 tsp_status enqueue_batch_step(tsp_batch *b) {
     int &OLD = b->role[TSP_ROLE_OLD], &NEW = b->role[TSP_ROLE_NEW], &ARG = b->role[TSP_ROLE_ARG],
         &ARGN = b->role[TSP_ROLE_ARG_NEXT];
@@ -167,7 +177,7 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
         if (stage > 0) std::swap(ARG, ARGN);
         if (b->level) // FillPatch of the stage's source register across the boxes
             TSP_HIP(tsp::launch_level_exchange(b->largs, stage == 0 ? OLD : ARG, b->max_shell, b->stream));
-        for (int v = 0; v < tsp::kBatchVariants; ++v) {
+        for (int v = 0; v < tsp::kBatchGroups; ++v) {
             if (b->group_n[v] == 0) continue;
             tsp::BatchArgs a{};
             a.inst = b->d_inst;
@@ -181,7 +191,7 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
             a.zalt = b->cfg.zalt;
             a.ghost_mode = b->level ? 1 : 0; // a box's ghosts hold its neighbours' values after the exchange
             const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-            TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
+            TSP_HIP(tsp::launch_batched_group(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
         }
     }
     std::swap(OLD, NEW);
@@ -192,7 +202,7 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
 tsp_status enqueue_batch_euler_step(tsp_batch *b) {
     int &OLD = b->role[TSP_ROLE_OLD], &NEW = b->role[TSP_ROLE_NEW];
     if (b->level) TSP_HIP(tsp::launch_level_exchange(b->largs, OLD, b->max_shell, b->stream));
-    for (int v = 0; v < tsp::kBatchVariants; ++v) {
+    for (int v = 0; v < tsp::kBatchGroups; ++v) {
         if (b->group_n[v] == 0) continue;
         tsp::BatchArgs a{};
         a.inst = b->d_inst;
@@ -206,7 +216,7 @@ tsp_status enqueue_batch_euler_step(tsp_batch *b) {
         a.zalt = b->cfg.zalt;
         a.ghost_mode = b->level ? 1 : 0;
         const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-        TSP_HIP(tsp::launch_batched_phipi(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
+        TSP_HIP(tsp::launch_batched_group(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
     }
     std::swap(OLD, NEW);
     return TSP_OK;
@@ -281,8 +291,9 @@ tsp_status create_batch(const tsp_desc *descs, int32_t count, void *stream, tsp_
     for (int i = 0; i < count; ++i) {
         tsp_status st = box_off ? TSP_OK : validate(&descs[i]);
         if (st != TSP_OK) return fail(st, "instance " + std::to_string(i) + ": " + g_err);
-        if (descs[i].system != TSP_SYS_WAVE_PHIPI)
-            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": batched mode runs the (phi,Pi) system");
+        if (tsp::batch_system_slot(descs[i].system) < 0)
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) +
+                                             ": batched mode runs the (phi,Pi), (phi,u) and NonLinear systems");
         if (tsp::batch_variant(descs[i].stencil_n, descs[i].ko_order) < 0)
             return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) + ": no batched kernel for stencil_n=" +
                                              std::to_string(descs[i].stencil_n) +
@@ -351,7 +362,7 @@ tsp_status create_batch(const tsp_desc *descs, int32_t count, void *stream, tsp_
         I.sj = (int32_t)L.sj;
         I.sk = (int32_t)L.sk;
         I.sc = (int32_t)L.sc;
-        I.variant = tsp::batch_variant(d.stencil_n, d.ko_order);
+        I.variant = tsp::batch_group(d.system, d.stencil_n, d.ko_order);
         for (int a = 0; a < 3; ++a) I.periodic[a] = d.periodic[a];
         I.dirichlet = d.bc == TSP_BC_DIRICHLET;
         I.abl_T = d.abl_thickness;
@@ -599,8 +610,10 @@ static std::string batch_tune_key(const tsp_batch *b) {
     int64_t cells = 0;
     for (const tsp_desc &d : b->d) cells += (int64_t)d.n[0] * d.n[1] * d.n[2];
     std::string groups;
-    for (int v = 0; v < tsp::kBatchVariants; ++v)
-        if (b->group_n[v] > 0) groups += "n" + std::to_string(tsp::kBatchN[v]) + "r" + std::to_string(tsp::kBatchR[v]);
+    for (int v = 0; v < tsp::kBatchGroups; ++v)
+        if (b->group_n[v] > 0)
+            groups += "s" + std::to_string(v / tsp::kBatchVariants) + "n" + std::to_string(tsp::kBatchN[v % tsp::kBatchVariants]) +
+                      "r" + std::to_string(tsp::kBatchR[v % tsp::kBatchVariants]);
     char buf[256];
     std::snprintf(buf, sizeof(buf), "%s|%s|%d|%lld|%s", arch.c_str(), b->level ? "level" : "batch", b->count,
                   (long long)cells, groups.c_str());
@@ -764,8 +777,8 @@ tsp_status tsp_level_create(const tsp_desc *d, const int32_t *block, void *strea
     if (!block) return fail(TSP_INVALID_ARG, "block size is NULL");
     tsp_status st = validate(d);
     if (st != TSP_OK) return st;
-    if (d->system != TSP_SYS_WAVE_PHIPI)
-        return fail(TSP_UNSUPPORTED, "multi-box levels run the (phi,Pi) system");
+    if (tsp::batch_system_slot(d->system) < 0)
+        return fail(TSP_UNSUPPORTED, "multi-box levels run the (phi,Pi), (phi,u) and NonLinear systems");
     for (int a = 0; a < 3; ++a)
         if (block[a] < 1) return fail(TSP_INVALID_ARG, "block size entries must be >= 1");
     std::vector<int> cuts[3];
@@ -1105,6 +1118,11 @@ tsp_status tsp_batch_adjoint_create(tsp_batch *b, const int32_t *receivers, int3
     if (nrec < 1 || !receivers) return fail(TSP_INVALID_ARG, "need at least one receiver");
     if (nsteps < 1) return fail(TSP_INVALID_ARG, "nsteps must be >= 1");
     if (checkpoint_every < 1) return fail(TSP_INVALID_ARG, "checkpoint_every must be >= 1");
+    if (b->level) return fail(TSP_UNSUPPORTED, "the batched adjoint runs batches of single boxes, not a level");
+    for (int i = 0; i < b->count; ++i)
+        if (b->d[i].system != TSP_SYS_WAVE_PHIPI)
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) +
+                                             ": the discrete adjoint is available for the (phi,Pi) system");
     std::vector<int> off((size_t)b->count * nrec);
     for (int i = 0; i < b->count; ++i)
         for (int r = 0; r < nrec; ++r) {

@@ -1,7 +1,8 @@
 // batch_kernel.hpp -- one launch per RK4 stage for many independent boxes.
 //
-// Every instance is a separate (phi, Pi) model with its own box size, stencil
-// width, KO order and strength, boundary handling and time step; all step in
+// Every instance is a separate model of the (phi, Pi), (phi, u) or NonLinear
+// (phi, pi) system with its own box size, stencil width, KO order and
+// strength, boundary handling and time step; all step in
 // lockstep, so the register roles (state_old, state_new, arg, arg_next) rotate
 // identically for all of them and travel as kernel arguments. The work list
 // holds (instance, tile, z chunk) items, largest instances first; a grid of
@@ -19,7 +20,7 @@ struct BatchInstance {
     double *reg[4];          // register origins (element (0,0,0,0))
     const double *sigma_tab; // ABL rate table or null
     int32_t nx, ny, nz, G, sj, sk, sc;
-    int32_t variant;         // index into the batched (stencil, KO order) list
+    int32_t variant;         // work-list group: batch_group(system, stencil width, KO order)
     int32_t periodic[3];
     int32_t dirichlet, abl_T, zchunk;
     int32_t off[3], dom[3]; // a box of a level: its offset in the domain and the domain's cells per axis
@@ -53,6 +54,18 @@ inline int batch_variant(int n, int r) {
     for (int v = 0; v < kBatchVariants; ++v)
         if (kBatchN[v] == n && kBatchR[v] == r) return v;
     return -1;
+}
+
+// systems the batched kernels carry; a work-list group is one (system, variant)
+// pair, group = slot * kBatchVariants + variant with the (phi, Pi) system at slot 0
+constexpr int kBatchSystems = 3;
+constexpr int kBatchGroups = kBatchSystems * kBatchVariants;
+inline int batch_system_slot(int system) {
+    return system == TSP_SYS_WAVE_PHIPI ? 0 : system == TSP_SYS_WAVE_PHIU ? 1 : system == TSP_SYS_NONLINEAR ? 2 : -1;
+}
+inline int batch_group(int system, int n, int r) {
+    const int s = batch_system_slot(system), v = batch_variant(n, r);
+    return s < 0 || v < 0 ? -1 : s * kBatchVariants + v;
 }
 
 // Work item of block slot w. The hardware deals blocks to the 8 XCDs round
@@ -200,19 +213,29 @@ hipError_t launch_batched(const BatchArgs &b, int tile_y, int grid, int max_abl_
     }
 }
 
-// every (stencil width, KO order) variant of the (phi, Pi) batch for one
-// stage body; instantiated in batch.hip (SM = -1), batch_s0.hip and
-// batch_s3.hip, which the build compiles in parallel
-template <int SM>
-hipError_t launch_batched_phipi_mode(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
-                                     int halo_ring, hipStream_t stream) {
+// every (stencil width, KO order) variant of one system's batch for one stage
+// body; instantiated in batch.hip ((phi, Pi), SM = -1), batch_s0.hip and
+// batch_s3.hip ((phi, Pi) stage-specific bodies), batch_phiu.hip and
+// batch_nonlinear.hip (generic body), which the build compiles in parallel
+template <class Sys, int SM>
+hipError_t launch_batched_mode(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                               hipStream_t stream) {
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kBatchVariants>([&]<int V>() {
         if (variant == V)
-            e = launch_batched<SysPhiPi, kBatchN[V], kBatchR[V], SM>(b, tile_y, grid, max_abl_T, halo_ring, stream);
+            e = launch_batched<Sys, kBatchN[V], kBatchR[V], SM>(b, tile_y, grid, max_abl_T, halo_ring, stream);
     });
     return e;
 }
+template <int SM>
+hipError_t launch_batched_phipi_mode(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                     int halo_ring, hipStream_t stream) {
+    return launch_batched_mode<SysPhiPi, SM>(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
+}
+hipError_t launch_batched_phiu(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                               hipStream_t stream);
+hipError_t launch_batched_nonlinear(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T,
+                                    int halo_ring, hipStream_t stream);
 hipError_t launch_batched_phipi_s0(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                    hipStream_t stream);
 hipError_t launch_batched_phipi_s3(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
@@ -220,6 +243,9 @@ hipError_t launch_batched_phipi_s3(int variant, const BatchArgs &b, int tile_y, 
 
 // launch one variant group of the (phi, Pi) batch
 hipError_t launch_batched_phipi(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                                hipStream_t stream);
+// launch one work-list group (system and variant) of a batch
+hipError_t launch_batched_group(int group, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
                                 hipStream_t stream);
 // one reverse stage (b.stage = j) of the discrete adjoint for one variant group
 // (transposed system on the adjoint registers; batch_adjoint.hip)

@@ -1,0 +1,11 @@
+// batch_nonlinear.hip -- batched stage kernels of the NonLinear (phi, pi) system for every
+// variant (generic stage body); a translation unit of its own so the build
+// compiles it in parallel.
+#include "batch_kernel.hpp"
+
+namespace tsp {
+hipError_t launch_batched_nonlinear(int variant, const BatchArgs &b, int tile_y, int grid, int max_abl_T, int halo_ring,
+                                    hipStream_t stream) {
+    return launch_batched_mode<SysNonLinear, -1>(variant, b, tile_y, grid, max_abl_T, halo_ring, stream);
+}
+} // namespace tsp

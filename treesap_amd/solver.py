@@ -170,7 +170,8 @@ class Solver:
 
 
 class Batch:
-    """Many independent (phi, Pi) models advanced in lockstep by one launch per RK4 stage."""
+    """Many independent (phi, Pi), (phi, u) and NonLinear models advanced in lockstep by one launch per
+    RK4 stage and (system, stencil width, KO order) group."""
 
     def __init__(self, cases, stream=None):
         self.cases = list(cases)
@@ -294,7 +295,7 @@ class Batch:
 class Level(Batch):
     """One level split into boxes of at most `block` cells per axis (the BoxArray of
     SingleLevelIntegrator(model, block_size), SingleLevelIntegrator.hpp:47-49,172-176), each box its own
-    registers, ghosts exchanged between boxes before every stage; (phi, Pi) system. Results are
+    registers, ghosts exchanged between boxes before every stage; (phi, Pi), (phi, u) or NonLinear. Results are
     bit-identical to one box of the whole domain."""
 
     def __init__(self, case: Case, block, stream=None):
