@@ -354,6 +354,12 @@ tsp_status tsp_batch_get_stream(tsp_batch *b, void **stream);
  * mu_{j-1} = b_{j-1} lambda' + a_j nu_j, lambda = lambda' + sum_j nu_j
  * (each reverse stage is one fused stage-kernel launch on the transpose
  * operator, with zero ghost values on non-periodic faces).
+ * For the NonLinear (phi, pi) system (examples/02-temporaries/NonLinearRHS.hpp:52-71,
+ * stencil_n 1..4) G^T is the transpose of the step's Jacobian at the forward
+ * state: each reverse stage j adds N(a_j)^T mu_j to nu_j, N the linearised
+ * -gamma pi |grad phi|^2 at the step's stage argument a_j, which the backward
+ * pass recomputes from the stored forward states (two more launches per
+ * stage, adjoint_nl.hip).
  * An adjoint object is bound to one solver (borrowed; it must outlive it) and
  * a fixed receiver set and step count; `receivers` holds nrec (i, j, k) valid
  * cell indices. */
@@ -374,11 +380,15 @@ tsp_status tsp_adjoint_checkpoint(tsp_adjoint *a, int32_t slot, double *dst, int
  * components, valid layout) to grad. With recompute != 0 every segment's
  * forward states are recomputed from its checkpoint before it is reversed
  * (the solver is left at the end of the forward pass); the linear system's
- * transpose does not depend on them, so the default skips that work. */
+ * transpose does not depend on them, so the default skips that work. The
+ * NonLinear system always recomputes: each segment's states are kept (one
+ * segment, checkpoint_every states, at a time) and every reverse step
+ * recomputes its stage arguments from them. */
 tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dtrace_on_device, double *grad,
                                 int32_t grad_on_device, int32_t recompute);
 /* One transposed RK4 step, lam_out = G^T lam_in (valid layout), for
- * dot-product checks. */
+ * dot-product checks; for the NonLinear system linearised at the solver's
+ * current state. */
 tsp_status tsp_adjoint_apply(tsp_adjoint *a, const double *lam_in, double *lam_out, int32_t on_device);
 
 /* Discrete adjoint of a whole batch (no reference counterpart): the single-box

@@ -707,6 +707,155 @@ int or_adjoint_step(or_solver *s, double *lam) {
     return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* discrete adjoint of one RK4 step, NonLinear (phi, pi) system               */
+/* ------------------------------------------------------------------------- */
+
+/* examples/02-temporaries/NonLinearRHS.hpp:52-71: F_phi = pi, F_pi = Lap phi -
+ * gamma pi |grad phi|^2 (+ KO, ABL). Its Jacobian at a state (phi_b, pi_b) is
+ * the (phi, Pi) operator L plus N with
+ *   (N v)_pi = -gamma (v_pi |g|^2 + 2 pi_b g . D1(v_phi)),  g = D1(phi_b),
+ * so J^T mu = L^T mu + N^T mu,
+ *   (N^T mu)_pi  = -(gamma |g|^2) mu_pi,
+ *   (N^T mu)_phi = -sum_a gf1_a sum_s w1(n,s) (W_a(i - s e_a) - W_a(i + s e_a)),
+ *   W_a = ((2 gamma) pi_b) g_a mu_pi  (zero outside the valid box along
+ *   non-periodic axes; periodic images along periodic ones).
+ * Each reverse stage j applies L^T as or_adjoint_step does, then adds N^T at
+ * the stage argument a_j of the forward step (a_0 = u, a_j = u + a_j k_{j-1})
+ * to nu_j in both of its uses. */
+static double nl_w_at(const double *W, const or_solver *s, int a, int i, int j, int k) {
+    int idx[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+        if (idx[d] < 0 || idx[d] >= s->d.n[d]) {
+            if (!s->d.periodic[d]) return 0.0;
+            idx[d] = (idx[d] % s->d.n[d] + s->d.n[d]) % s->d.n[d];
+        }
+    }
+    return W[(((int64_t)a * s->d.n[2] + idx[2]) * s->d.n[1] + idx[1]) * s->d.n[0] + idx[0]];
+}
+
+/* The forward step's stage arguments a_0..a_3 from the state u (valid region),
+ * ghosts filled as the forward stages read them: st[q] = register of a_q. */
+static void nl_stage_args(const or_solver *s, const double *u, double **st) {
+    or_solver *f = or_create(&s->d);
+    or_set_state(f, u);
+    const double dt = s->d.dt;
+    const size_t bytes = sizeof(double) * (size_t)f->gcells * f->nc;
+    fill_ghosts(f, f->r_old);
+    memcpy(st[0], f->reg[f->r_old], bytes);
+    for (int q = 0; q < 4; ++q) {
+        rk4_stage(f, q, dt);
+        if (q > 0) memcpy(st[q], f->reg[f->r_arg], bytes); /* the stage's source, ghosts filled */
+    }
+    or_destroy(f);
+}
+
+int or_adjoint_step_nl(or_solver *s, double *lam, const double *u) {
+    if (s->d.system != 2) return -1;
+    const double dt = s->d.dt, gamma = s->d.gamma;
+    const int nx = s->d.n[0], ny = s->d.n[1], nz = s->d.n[2], n = s->d.stencil_n;
+    const int64_t cells = (int64_t)nx * ny * nz;
+    const size_t rbytes = sizeof(double) * (size_t)s->gcells * s->nc;
+    double *st[4];
+    for (int q = 0; q < 4; ++q) st[q] = (double *)malloc(rbytes);
+    double *W = (double *)malloc(sizeof(double) * (size_t)cells * 4);
+    nl_stage_args(s, u, st);
+    const int LP = 0, ACC = 1, MA = 2, MB = 3;
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double v = lam[(((int64_t)c * nz + k) * ny + j) * nx + i];
+                    AT(s, LP, i, j, k, c) = v;
+                    AT(s, MA, i, j, k, c) = (dt * b_frac[3]) * v;
+                }
+    int src = MA, dst = MB;
+    for (int stage = 3; stage >= 0; --stage) {
+        fill_ghosts_homogeneous(s, src);
+        const double bprev = stage > 0 ? dt * b_frac[stage - 1] : 0.0;
+        const double astage = dt * a_frac[stage];
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int c = 0; c < 2; ++c) {
+                        double nu = adj_rhs(s, src, c, i, j, k);
+                        double lp = AT(s, LP, i, j, k, c);
+                        if (stage == 3)
+                            AT(s, ACC, i, j, k, c) = lp + nu;
+                        else
+                            AT(s, ACC, i, j, k, c) = AT(s, ACC, i, j, k, c) + nu;
+                        if (stage > 0) AT(s, dst, i, j, k, c) = bprev * lp + astage * nu;
+                    }
+        /* N^T at the stage argument: W and the pi part from mu (src) and a_stage */
+        double *areg = st[stage];
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int64_t q = ((int64_t)k * ny + j) * nx + i;
+                    double g[3];
+                    for (int a = 0; a < 3; ++a) {
+                        double sum = 0.0;
+                        for (int t = 1; t <= n; ++t) {
+                            int di, dj, dk;
+                            step_dir(a, t, &di, &dj, &dk);
+                            const double fp = areg[((int64_t)(i + di) + s->G) + ((int64_t)(j + dj) + s->G) * s->gx +
+                                                   ((int64_t)(k + dk) + s->G) * s->gx * s->gy];
+                            const double fm = areg[((int64_t)(i - di) + s->G) + ((int64_t)(j - dj) + s->G) * s->gx +
+                                                   ((int64_t)(k - dk) + s->G) * s->gx * s->gy];
+                            sum += or_w1(n, t) * (fp - fm);
+                        }
+                        g[a] = (1.0 / s->dx[a]) * sum;
+                    }
+                    double gsq = g[0] * g[0];
+                    gsq += g[1] * g[1];
+                    gsq += g[2] * g[2];
+                    const double pib = areg[((int64_t)i + s->G) + ((int64_t)j + s->G) * s->gx +
+                                            ((int64_t)k + s->G) * s->gx * s->gy + s->gcells];
+                    const double mupi = AT(s, src, i, j, k, 1);
+                    const double tp = (2.0 * gamma) * pib;
+                    for (int a = 0; a < 3; ++a) W[a * cells + q] = (tp * g[a]) * mupi;
+                    W[3 * cells + q] = -((gamma * gsq) * mupi);
+                }
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double cphi = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        double sum = 0.0;
+                        for (int t = 1; t <= n; ++t) {
+                            int di, dj, dk;
+                            step_dir(a, t, &di, &dj, &dk);
+                            sum += or_w1(n, t) * (nl_w_at(W, s, a, i - di, j - dj, k - dk) -
+                                                  nl_w_at(W, s, a, i + di, j + dj, k + dk));
+                        }
+                        cphi += (1.0 / s->dx[a]) * sum;
+                    }
+                    const double c_phi = -cphi;
+                    const double c_pi = W[3 * cells + ((int64_t)k * ny + j) * nx + i];
+                    AT(s, ACC, i, j, k, 0) = AT(s, ACC, i, j, k, 0) + c_phi;
+                    AT(s, ACC, i, j, k, 1) = AT(s, ACC, i, j, k, 1) + c_pi;
+                    if (stage > 0) {
+                        AT(s, dst, i, j, k, 0) = AT(s, dst, i, j, k, 0) + astage * c_phi;
+                        AT(s, dst, i, j, k, 1) = AT(s, dst, i, j, k, 1) + astage * c_pi;
+                    }
+                }
+        int t = src;
+        src = dst;
+        dst = t;
+    }
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    lam[(((int64_t)c * nz + k) * ny + j) * nx + i] = AT(s, ACC, i, j, k, c);
+    for (int q = 0; q < 4; ++q) free(st[q]);
+    free(W);
+    return 0;
+}
+
 /* OpenMP threads used by the stepping loops (CPU-baseline timing) */
 void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int or_max_threads(void) { return omp_get_max_threads(); }

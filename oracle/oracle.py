@@ -107,6 +107,8 @@ def lib() -> C.CDLL:
         h.or_ko_coef.argtypes = [C.c_int, C.c_int]
         h.or_adjoint_step.restype = C.c_int
         h.or_adjoint_step.argtypes = [VP, DP]
+        h.or_adjoint_step_nl.restype = C.c_int
+        h.or_adjoint_step_nl.argtypes = [VP, DP, DP]
         h.or_set_threads.argtypes = [C.c_int]
         h.or_max_threads.restype = C.c_int
         h.or_system_ncomp.restype = C.c_int
@@ -213,6 +215,16 @@ class OracleSolver:
         assert a.shape == self.shape
         if lib().or_adjoint_step(self._h, _dp(a)) != 0:
             raise ValueError("adjoint step needs the (phi, Pi) system")
+        return a
+
+    def adjoint_step_nl(self, lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+        """G'(u)^T lam for one RK4 step of the NonLinear (phi, pi) system linearised at the state u
+        (registers are used as scratch)."""
+        a = np.array(lam, dtype=np.float64, copy=True, order="C")
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        assert a.shape == self.shape and uu.shape == self.shape
+        if lib().or_adjoint_step_nl(self._h, _dp(a), _dp(uu)) != 0:
+            raise ValueError("nonlinear adjoint step needs the NonLinear system")
         return a
 
     def point_derivative(self, dirs, i, j, k, c=0):

@@ -276,3 +276,103 @@ def test_adjoint_rejects_bad_tensors():
     a.trace(out=good)
     a.close()
     s.close()
+
+
+# ---- NonLinear (phi, pi) system: G'(u)^T, linearised at the forward states ----
+
+NL_CASES = [  # n, stencil, KO order, periodic, ABL, gamma
+    ((26, 20, 18), 2, 2, (1, 1, 1), 0, 0.3),
+    ((30, 22, 20), 3, 3, (0, 0, 0), 3, 0.5),
+    ((21, 17, 19), 1, 0, (1, 0, 1), 0, 0.2),
+    ((70, 21, 19), 4, 4, (0, 0, 0), 4, 0.4),
+]
+
+
+def _nl_case(n, sn, ko, per, abl, gamma, seed=8):
+    case, st = small_case("nonlinear", n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=seed, gamma=gamma)
+    return case, st
+
+
+@pytest.mark.parametrize("n,sn,ko,per,abl,gamma", NL_CASES)
+def test_nonlinear_transposed_step_matches_oracle(n, sn, ko, per, abl, gamma):
+    """tsp_adjoint_apply on the NonLinear system is the transpose of the step's Jacobian at the
+    solver's state, bitwise against the oracle's (itself pinned against a numpy tangent-linear
+    model), for two states and two adjoint vectors."""
+    _say(f"[adjoint] nonlinear apply n={n} stencil={sn} ko={ko} periodic={per} abl={abl}")
+    case, st = _nl_case(n, sn, ko, per, abl, gamma)
+    s = Solver(case)
+    a = Adjoint(s, [[1, 1, 1]], 1, 1)
+    rng = np.random.default_rng(9)
+    o = orc.OracleSolver(case)
+    for q in range(2):
+        u = st if q == 0 else st + 0.3 * rng.standard_normal(case.shape)
+        s.set_state(u)
+        lam = rng.standard_normal(case.shape)
+        got = a.apply(lam)
+        assert np.array_equal(got, o.adjoint_step_nl(lam, u)), q
+        assert np.array_equal(s.get_state(), u)  # the forward state is untouched
+
+
+def _nl_problem(n=(30, 26, 22), sn=2, ko=2, per=(0, 0, 0), abl=4, gamma=0.4, nsteps=8, K=3, nrec=6, seed=12):
+    case, st = _nl_case(n, sn, ko, per, abl, gamma, seed=seed)
+    rng = np.random.default_rng(seed)
+    rec = np.stack([rng.integers(0, n[a], size=nrec) for a in range(3)], axis=1).astype(np.int32)
+    rec[1] = rec[0]
+    target = rng.normal(0.0, 0.01, size=(nsteps, nrec))
+    return case, st, rec, target, nsteps, K
+
+
+def _oracle_trace_and_grad_nl(case, st, rec, target, nsteps):
+    o = orc.OracleSolver(case)
+    o.set_state(st)
+    states, trace = [], np.empty((nsteps, len(rec)))
+    for n in range(nsteps):
+        states.append(o.get_state())
+        o.rk4_steps(1)
+        phi = o.get_state()[0]
+        trace[n] = [phi[k, j, i] for i, j, k in rec]
+    dtrace = 2.0 * (trace - target)
+    lam = np.zeros(o.shape)
+    adj = orc.OracleSolver(case)
+    for n in range(nsteps, 0, -1):
+        for r, (i, j, k) in enumerate(rec):
+            lam[0, k, j, i] += dtrace[n - 1, r]
+        lam = adj.adjoint_step_nl(lam, states[n - 1])
+    return trace, dtrace, lam
+
+
+@pytest.mark.parametrize("ko,per,abl,gamma", [(2, (0, 0, 0), 4, 0.4), (3, (1, 1, 1), 0, 0.6), (0, (1, 0, 1), 3, 0.3)])
+def test_nonlinear_backward_matches_oracle_and_finite_differences(ko, per, abl, gamma):
+    """The NonLinear backward pass (checkpoints every 3 of 8 steps, each segment's states
+    recomputed and every reverse step linearised at its own state) equals the oracle's reverse
+    sweep bitwise, leaves the solver at the end of the forward pass, and its gradient matches
+    centred finite differences of the loss (eps = 1e-6) in a random direction."""
+    _say(f"[adjoint] nonlinear backward ko={ko} periodic={per} abl={abl} gamma={gamma}")
+    case, st, rec, target, nsteps, K = _nl_problem(ko=ko, per=per, abl=abl, gamma=gamma)
+    s = Solver(case)
+    s.set_state(st)
+    a = Adjoint(s, rec, nsteps, K)
+    a.forward()
+    final, t_final = s.get_state(), s.time
+    trace = a.trace()
+    otrace, odtrace, ograd = _oracle_trace_and_grad_nl(case, st, rec, target, nsteps)
+    assert np.array_equal(trace, otrace)
+    g = a.backward(odtrace)
+    assert np.array_equal(g, ograd)
+    assert np.array_equal(a.backward(odtrace, recompute=True), ograd)
+    assert np.array_equal(s.get_state(), final) and s.time == t_final
+
+    def loss(u):
+        t = Solver(case)
+        t.set_state(u)
+        tr = np.empty((nsteps, len(rec)))
+        for n in range(nsteps):
+            t.rk4_steps(1)
+            phi = t.get_state()[0]
+            tr[n] = [phi[k, j, i] for i, j, k in rec]
+        return float(np.sum((tr - target) ** 2))
+
+    v = np.random.default_rng(21).standard_normal(case.shape)
+    eps = 1e-6
+    fd = (loss(st + eps * v) - loss(st - eps * v)) / (2 * eps)
+    assert float(np.sum(g * v)) == pytest.approx(fd, rel=1e-6)

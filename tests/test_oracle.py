@@ -420,6 +420,138 @@ def test_oracle_adjoint_dot_product(n_st, ko, periodic, abl):
         assert lhs == pytest.approx(rhs, rel=1e-13, abs=1e-13 * np.sqrt(np.sum(gu * gu) * np.sum(v * v)))
 
 
+# ---------------------------------------------------------------------------
+# discrete adjoint of the NonLinear system: the oracle's G'(u)^T against an
+# independent numpy tangent-linear model of the RK4 step (padded-array
+# stencils), which is itself checked against centred differences of the
+# oracle's forward step
+# ---------------------------------------------------------------------------
+def _np_nonlinear_step_and_tangent(case, u, v, sigma):
+    """(G(u), G'(u) v) of one RK4 step of the decorated NonLinear system
+    F_phi = pi + KO - s (phi - t0), F_pi = Lap phi - gamma pi |D1 phi|^2 + KO - s (pi - t1)
+    (examples/02-temporaries/NonLinearRHS.hpp:52-71), in exact-arithmetic form."""
+    n = case["stencil_n"]
+    G = case["ghost"]
+    r = case["ko_order"]
+    gam = case["gamma"]
+    dx = [(case["prob_hi"][a] - case["prob_lo"][a]) / case["n"][a] for a in range(3)]
+    w1, w2, _ = orc.weights(n)
+    kc = orc.ko_coefs(r) if r else []
+    tgt = case["target"]
+    dirichlet = case["bc"] == "dirichlet"
+
+    def pad(f, value):
+        P = f
+        for a in range(3):  # x is numpy axis 2
+            ax = 2 - a
+            width = [(0, 0)] * 3
+            width[ax] = (G, G)
+            P = np.pad(P, width, mode="wrap") if case["periodic"][a] else np.pad(P, width, constant_values=value)
+        return P
+
+    def sh(P, a, q):
+        ax = 2 - a
+        sl = [slice(G, P.shape[d] - G) for d in range(3)]
+        sl[ax] = slice(G + q, P.shape[ax] - G + q)
+        return P[tuple(sl)]
+
+    def D1(P, a):
+        return sum(w1[q] * (sh(P, a, q) - sh(P, a, -q)) for q in range(1, n + 1)) / dx[a]
+
+    def D2(P, a):
+        return (w2[0] * sh(P, a, 0) + sum(w2[q] * (sh(P, a, q) + sh(P, a, -q)) for q in range(1, n + 1))) / dx[a] ** 2
+
+    def KO(P):
+        if not r:
+            return 0.0
+        tot = 0.0
+        for a in range(3):
+            res = kc[0] * sh(P, a, 0) + sum(kc[q] * (sh(P, a, q) + sh(P, a, -q)) for q in range(1, r + 1))
+            tot = tot + res / 2.0 ** (2 * r) / dx[a]
+        return case["ko_strength"] * tot
+
+    def F(w):
+        Pphi = pad(w[0], tgt[0] if dirichlet else 0.0)
+        Ppi = pad(w[1], tgt[1] if dirichlet else 0.0)
+        g = [D1(Pphi, a) for a in range(3)]
+        gsq = g[0] ** 2 + g[1] ** 2 + g[2] ** 2
+        fphi = w[1] + KO(Pphi) - sigma * (w[0] - tgt[0])
+        fpi = sum(D2(Pphi, a) for a in range(3)) - gam * w[1] * gsq + KO(Ppi) - sigma * (w[1] - tgt[1])
+        return np.stack([fphi, fpi]), g, gsq
+
+    def J(w, dw):
+        _, g, gsq = F(w)
+        Pd = pad(dw[0], 0.0)
+        Pdpi = pad(dw[1], 0.0)
+        dg = [D1(Pd, a) for a in range(3)]
+        dphi = dw[1] + KO(Pd) - sigma * dw[0]
+        dpi = (sum(D2(Pd, a) for a in range(3)) - gam * (dw[1] * gsq + 2.0 * w[1] * (g[0] * dg[0] + g[1] * dg[1] +
+                                                                                       g[2] * dg[2]))
+               + KO(Pdpi) - sigma * dw[1])
+        return np.stack([dphi, dpi])
+
+    dt = case["dt"]
+    a_next = [0.5 * dt, 0.5 * dt, dt]
+    b = [dt / 6, dt / 3, dt / 3, dt / 6]
+    a, da = u, v
+    out, dout = u.copy(), v.copy()
+    for q in range(4):
+        k, _, _ = F(a)
+        dk = J(a, da)
+        out = out + b[q] * k
+        dout = dout + b[q] * dk
+        if q < 3:
+            a, da = u + a_next[q] * k, v + a_next[q] * dk
+    return out, dout
+
+
+NL_ADJ_CASES = [  # stencil n, KO order, periodic, ABL thickness, gamma
+    (2, 2, (1, 1, 1), 0, 0.3), (3, 3, (0, 0, 0), 3, 0.5), (1, 0, (1, 0, 1), 0, 0.2), (4, 4, (1, 1, 1), 0, 0.4),
+    (2, 0, (0, 1, 0), 2, 0.6)]
+
+
+def _nl_case(n_st, ko, periodic, abl, gamma, n=(13, 11, 12)):
+    tgt = (0.05, -0.08) + (0.0,) * 10 if not all(periodic) else (0.0,) * 12
+    return _case(system="nonlinear", n=n, stencil_n=n_st, ghost=max(n_st, ko), ko_order=ko,
+                 ko_strength=0.15 if ko else 0.0, periodic=periodic, bc="dirichlet" if not all(periodic) else "noop",
+                 abl_thickness=abl, abl_max=2.0 if abl else 0.0, target=tgt, gamma=gamma, dt=0.25 / 13,
+                 prob_hi=(1.0, 0.9, 1.1))
+
+
+@pytest.mark.parametrize("n_st,ko,periodic,abl,gamma", NL_ADJ_CASES)
+def test_oracle_nonlinear_adjoint_against_tangent_linear_model(n_st, ko, periodic, abl, gamma):
+    case = _nl_case(n_st, ko, periodic, abl, gamma)
+    n = case["n"]
+    sigma = _sigma_brute(n, abl, 2.0, "quadratic") if abl else np.zeros(n[::-1])
+    rng = np.random.default_rng(100 + 10 * n_st + ko)
+    s = orc.OracleSolver(case)
+    u = 0.5 * rng.standard_normal(s.shape)
+    v = rng.standard_normal(s.shape)
+    lam = rng.standard_normal(s.shape)
+    # the numpy step equals the oracle's forward step (to rounding) ...
+    g_np, tlm = _np_nonlinear_step_and_tangent(case, u, v, sigma)
+    s.set_state(u)
+    s.rk4_steps(1)
+    assert np.max(np.abs(s.get_state() - g_np)) < 1e-12 * np.max(np.abs(g_np))
+    # ... its tangent equals centred differences of the oracle's forward step ...
+    eps = 1e-5
+    fd = []
+    for sgn in (1.0, -1.0):
+        t = orc.OracleSolver(case)
+        t.set_state(u + sgn * eps * v)
+        t.rk4_steps(1)
+        fd.append(t.get_state())
+    fd = (fd[0] - fd[1]) / (2 * eps)
+    assert np.max(np.abs(fd - tlm)) < 1e-7 * np.max(np.abs(tlm))
+    # ... and the oracle's adjoint is its transpose: <G'(u) v, lam> = <v, G'(u)^T lam>
+    adj = orc.OracleSolver(case).adjoint_step_nl(lam, u)
+    lhs, rhs = float(np.sum(tlm * lam)), float(np.sum(v * adj))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    # the check has teeth: the linear (phi, Pi) transpose alone misses the nonlinear term
+    lin = orc.OracleSolver(dict(case, system="phipi", gamma=0.0)).adjoint_step(lam)
+    assert abs(float(np.sum(v * lin)) - lhs) > 1e-6 * abs(lhs)
+
+
 def test_oracle_adjoint_is_not_the_forward():
     """The transpose differs from the forward step where the operator is not symmetric (the check has teeth)."""
     case = _case(n=(12, 10, 8), stencil_n=2, ghost=2, dt=0.03)

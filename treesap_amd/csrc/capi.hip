@@ -1090,6 +1090,14 @@ struct tsp_adjoint {
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
     tsp_kernel_cfg gcfg{-1, -1, -1, -1, -1, -1, -1};
+    // NonLinear system: the reverse step at forward step n needs that step's
+    // stage arguments: the state u_n (xbase[0]) and a_1..a_3 (xbase[1..3]),
+    // recomputed from the segment's stored states (d_seg); xbase[4] takes the
+    // stage accumulator, d_w the nonlinear terms' scratch
+    bool nl = false;
+    double *xbase[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *d_seg = nullptr; // K dense states of the segment being reversed
+    double *d_w = nullptr;   // 4 dense fields
 };
 
 namespace {
@@ -1097,6 +1105,32 @@ namespace {
 double *adj_origin(const tsp_adjoint *a, int reg) {
     const tsp_solver *s = a->s;
     return a->abase[reg] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
+}
+
+double *x_origin(const tsp_adjoint *a, int q) {
+    const tsp_solver *s = a->s;
+    return a->xbase[q] + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk;
+}
+
+// the forward step's stage arguments a_1..a_3 from u_n in xbase[0]: forward
+// stages 0..2 (the solver's own stage kernels, virtual ghosts) writing their
+// next argument into xbase[1..3] (SingleLevelRK4Integrator.hpp:137-156)
+tsp_status enqueue_nl_stage_args(tsp_adjoint *a) {
+    tsp_solver *s = a->s;
+    const double dt = s->d.dt;
+    for (int stage = 0; stage < 3; ++stage) {
+        tsp::StageArgs g = stage_args(s, stage, 0, 0, 0, 0, dt * kBFrac[stage], dt * kAFrac[stage + 1]);
+        g.src = x_origin(a, stage); // u_n, then a_1, a_2
+        g.init = x_origin(a, 0);
+        g.acc = x_origin(a, 4);
+        g.argn = x_origin(a, stage + 1);
+        g.ghost_mode = 0; // periodic images through the maps, Dirichlet ghosts written at creation
+        hipError_t e = hipErrorNotSupported;
+        if (s->launch_stage[stage]) e = s->launch_stage[stage](g, s->cfg.tile_y, s->stream);
+        if (e == hipErrorNotSupported) e = s->launch(g, s->cfg.tile_y, s->stream);
+        TSP_HIP(e);
+    }
+    return TSP_OK;
 }
 
 int ckpt_slot_of_step(const tsp_adjoint *a, int step) {
@@ -1147,6 +1181,29 @@ tsp_status enqueue_adjoint_step(tsp_adjoint *a) {
         g.ghost_mode = 0; // adjoint ghosts: periodic images or zero (prefilled)
         for (int c = 0; c < TSP_MAX_COMP; ++c) g.target[c] = 0.0;
         TSP_HIP(a->launch(g, s->cfg.tile_y, s->stream));
+        if (a->nl) { // + N^T mu_j at the forward stage argument a_j, in both uses of nu_j
+            tsp::NlAdjArgs q{};
+            q.mu = g.src;
+            q.mu_scale = g.src_scale;
+            q.abar = x_origin(a, j);
+            q.acc = g.acc;
+            q.argn = g.argn;
+            q.a_next = g.a_next;
+            q.w = a->d_w;
+            q.nx = s->d.n[0];
+            q.ny = s->d.n[1];
+            q.nz = s->d.n[2];
+            q.sj = (int32_t)s->sj;
+            q.sk = (int32_t)s->sk;
+            q.sc = (int32_t)s->sc;
+            q.n = s->d.stencil_n;
+            for (int d = 0; d < 3; ++d) {
+                q.periodic[d] = s->d.periodic[d];
+                q.gf1[d] = s->gf1[d];
+            }
+            q.gamma = s->d.gamma;
+            TSP_HIP(tsp::launch_nonlinear_adjoint_terms(q, s->stream));
+        }
     }
     std::swap(OLD, NEW);
     return TSP_OK;
@@ -1227,6 +1284,35 @@ tsp_status run_backward_range(tsp_adjoint *a, int n1, int n0, int *bparity) {
     return TSP_OK;
 }
 
+// NonLinear system: forward steps n0 -> n1 storing every state u_n (n0 <= n < n1)
+// in the segment buffer, with the receiver gather after every step
+tsp_status run_forward_range_storing(tsp_adjoint *a, int n0, int n1) {
+    tsp_solver *s = a->s;
+    for (int n = n0; n < n1; ++n) {
+        TSP_HIP(tsp::launch_pack(a->d_seg + (int64_t)(n - n0) * a->state_elems, reg_origin(s, s->role[TSP_ROLE_OLD]),
+                                 s->d.n[0], s->d.n[1], s->d.n[2], s->nc, (int)s->sj, (int)s->sk, (int)s->sc,
+                                 s->stream));
+        tsp_status st = enqueue_fwd_gather(a);
+        if (st != TSP_OK) return st;
+        advance_clock(s);
+    }
+    return TSP_OK;
+}
+
+// NonLinear system: reverse steps n1 -> n0, each linearised at its stored state
+// (counter must point at row n1 - 1)
+tsp_status run_backward_range_nl(tsp_adjoint *a, int n1, int n0) {
+    tsp_solver *s = a->s;
+    for (int n = n1; n > n0; --n) {
+        TSP_HIP(tsp::launch_unpack(x_origin(a, 0), a->d_seg + (int64_t)(n - 1 - n0) * a->state_elems, s->d.n[0],
+                                   s->d.n[1], s->d.n[2], s->nc, (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
+        tsp_status st = enqueue_nl_stage_args(a);
+        if (st == TSP_OK) st = enqueue_inject_adjoint(a);
+        if (st != TSP_OK) return st;
+    }
+    return TSP_OK;
+}
+
 tsp_status save_checkpoint(tsp_adjoint *a, int slot) {
     tsp_solver *s = a->s;
     TSP_HIP(tsp::launch_pack(a->d_ckpt + (int64_t)slot * a->state_elems, reg_origin(s, s->role[TSP_ROLE_OLD]), s->d.n[0],
@@ -1260,8 +1346,10 @@ tsp_status tsp_adjoint_create(tsp_solver *s, const int32_t *receivers, int32_t n
     if (!out) return fail(TSP_INVALID_ARG, "out is NULL");
     *out = nullptr;
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
-    if (s->d.system != TSP_SYS_WAVE_PHIPI)
-        return fail(TSP_UNSUPPORTED, "the discrete adjoint covers the linear (phi,Pi) system");
+    if (s->d.system != TSP_SYS_WAVE_PHIPI && s->d.system != TSP_SYS_NONLINEAR)
+        return fail(TSP_UNSUPPORTED, "the discrete adjoint covers the (phi,Pi) and NonLinear systems");
+    if (s->d.system == TSP_SYS_NONLINEAR && (s->d.stencil_n < 1 || s->d.stencil_n > 4))
+        return fail(TSP_UNSUPPORTED, "the NonLinear adjoint needs stencil_n in 1..4");
     if (nrec < 1 || !receivers) return fail(TSP_INVALID_ARG, "need at least one receiver");
     if (nsteps < 1) return fail(TSP_INVALID_ARG, "nsteps must be >= 1");
     if (checkpoint_every < 1) return fail(TSP_INVALID_ARG, "checkpoint_every must be >= 1");
@@ -1298,6 +1386,22 @@ tsp_status tsp_adjoint_create(tsp_solver *s, const int32_t *receivers, int32_t n
         // adjoint registers: ghosts are zero (homogeneous boundary values) for good
         if (e == hipSuccess) e = hipMemsetAsync(a->abase[r], 0, sizeof(double) * s->reg_elems, s->stream);
     }
+    a->nl = s->d.system == TSP_SYS_NONLINEAR;
+    if (a->nl) {
+        // stage-argument registers: the forward registers' ghosts (zero memory, Dirichlet values)
+        for (int q = 0; q < 5 && e == hipSuccess; ++q) {
+            e = hipMalloc((void **)&a->xbase[q], sizeof(double) * s->reg_elems);
+            if (e == hipSuccess) e = hipMemsetAsync(a->xbase[q], 0, sizeof(double) * s->reg_elems, s->stream);
+            if (e == hipSuccess) {
+                tsp::GhostArgs g = ghost_args(s, 0);
+                g.p = x_origin(a, q);
+                e = tsp::launch_ghost_fill(g, s->stream);
+            }
+        }
+        if (e == hipSuccess) e = hipMalloc((void **)&a->d_seg, sizeof(double) * a->state_elems * checkpoint_every);
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&a->d_w, sizeof(double) * 4 * (size_t)s->d.n[0] * s->d.n[1] * s->d.n[2]);
+    }
     if (e == hipSuccess) e = hipMemsetAsync(a->d_trace, 0, sizeof(double) * nrec * (size_t)nsteps, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e == hipErrorOutOfMemory) return cleanup(fail(TSP_OOM, "adjoint buffers do not fit in device memory"));
@@ -1311,7 +1415,9 @@ tsp_status tsp_adjoint_destroy(tsp_adjoint *a) {
     if (a->s && a->s->stream) (void)hipStreamSynchronize(a->s->stream);
     drop_adjoint_graphs(a);
     for (void *p : {(void *)a->d_rec, (void *)a->d_trace, (void *)a->d_dtrace, (void *)a->d_counter, (void *)a->d_ckpt,
-                    (void *)a->abase[0], (void *)a->abase[1], (void *)a->abase[2], (void *)a->abase[3]})
+                    (void *)a->abase[0], (void *)a->abase[1], (void *)a->abase[2], (void *)a->abase[3],
+                    (void *)a->xbase[0], (void *)a->xbase[1], (void *)a->xbase[2], (void *)a->xbase[3],
+                    (void *)a->xbase[4], (void *)a->d_seg, (void *)a->d_w})
         if (p) (void)hipFree(p);
     delete a;
     return TSP_OK;
@@ -1377,7 +1483,7 @@ tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dt
     TSP_HIP(tsp::launch_zero(a->abase[0], s->reg_elems, s->stream)); // lambda = 0 (ghosts stay zero)
     int bparity = 0;
     tsp_status st = TSP_OK;
-    if (!recompute) {
+    if (!recompute && !a->nl) {
         TSP_HIP(tsp::launch_set_counter(a->d_counter, a->nsteps - 1, s->stream));
         st = run_backward_range(a, a->nsteps, 0, &bparity);
     } else {
@@ -1399,12 +1505,12 @@ tsp_status tsp_adjoint_backward(tsp_adjoint *a, const double *dtrace, int32_t dt
                 hipError_t e = tsp::launch_set_counter(a->d_counter, n0, s->stream);
                 if (e != hipSuccess) st = hip_fail(e, "set counter");
             }
-            if (st == TSP_OK) st = run_forward_range(a, n0, n1);
+            if (st == TSP_OK) st = a->nl ? run_forward_range_storing(a, n0, n1) : run_forward_range(a, n0, n1);
             if (st == TSP_OK) {
                 hipError_t e = tsp::launch_set_counter(a->d_counter, n1 - 1, s->stream);
                 if (e != hipSuccess) st = hip_fail(e, "set counter");
             }
-            if (st == TSP_OK) st = run_backward_range(a, n1, n0, &bparity);
+            if (st == TSP_OK) st = a->nl ? run_backward_range_nl(a, n1, n0) : run_backward_range(a, n1, n0, &bparity);
         }
         // leave the forward solver at the end of the forward pass
         if (st == TSP_OK) st = restore_checkpoint(a, a->nck - 1, a->nsteps, saved_time);
@@ -1446,7 +1552,14 @@ tsp_status tsp_adjoint_apply(tsp_adjoint *a, const double *lam_in, double *lam_o
     std::memcpy(a->arole, (const int[4]){0, 1, 2, 3}, sizeof(a->arole));
     hipError_t e = tsp::launch_unpack(adj_origin(a, a->arole[TSP_ROLE_OLD]), src, nx, ny, nz, s->nc, (int)s->sj,
                                       (int)s->sk, (int)s->sc, s->stream);
-    tsp_status st = e == hipSuccess ? enqueue_adjoint_step(a) : hip_fail(e, "unpack");
+    tsp_status st = e == hipSuccess ? TSP_OK : hip_fail(e, "unpack");
+    if (st == TSP_OK && a->nl) { // linearised at the solver's current state
+        // the whole register, ghosts included (the same boundary values)
+        e = hipMemcpyAsync(a->xbase[0], s->base[s->role[TSP_ROLE_OLD]], sizeof(double) * (size_t)s->reg_elems,
+                           hipMemcpyDeviceToDevice, s->stream);
+        st = e == hipSuccess ? enqueue_nl_stage_args(a) : hip_fail(e, "state copy");
+    }
+    if (st == TSP_OK) st = enqueue_adjoint_step(a);
     if (st == TSP_OK) {
         double *dst = on_device ? lam_out : staging;
         e = tsp::launch_pack(dst, adj_origin(a, a->arole[TSP_ROLE_OLD]), nx, ny, nz, s->nc, (int)s->sj, (int)s->sk,

@@ -172,6 +172,24 @@ struct GhostArgs {
 };
 hipError_t launch_ghost_fill(const GhostArgs &g, hipStream_t stream);
 
+// nonlinear terms of one reverse stage of the NonLinear system's discrete
+// adjoint (adjoint_nl.hip): acc += N^T mu, argn += a_next N^T mu, N the
+// linearised -gamma pi |grad phi|^2 at the stage argument abar
+struct NlAdjArgs {
+    const double *mu;   // mu_j, register origin (element (0,0,0,0))
+    double mu_scale;    // mu_j = mu_scale * register (b_3 at the last stage, else 1)
+    const double *abar; // the forward stage argument a_j (phi, pi), ghosts as the forward stage read them
+    double *acc;        // sum of the stage terms
+    double *argn;       // the next stage's mu (null at stage 0)
+    double a_next;      // a_j
+    double *w;          // 4 dense scratch fields over the valid cells
+    int32_t nx, ny, nz, sj, sk, sc, n;
+    int32_t periodic[3];
+    double gf1[3];
+    double gamma;
+};
+hipError_t launch_nonlinear_adjoint_terms(const NlAdjArgs &a, hipStream_t stream);
+
 // valid-region pack/unpack between a dense [c][k][j][i] buffer and a register
 hipError_t launch_unpack(double *reg0, const double *dense, int nx, int ny, int nz, int nc, int sj, int sk, int sc,
                          hipStream_t stream);
