@@ -134,7 +134,7 @@ template <int N, int R> struct TensorTile {
         double *g = lds + REG;
         const int tid = threadIdx.x;
         const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-        const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+        const int tiles_x = (a.nx + TX - 1) / TX;
         // blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous
         // run of tiles so that neighbours share an L2 for their common halo
         const int nb = gridDim.x;
@@ -143,9 +143,12 @@ template <int N, int R> struct TensorTile {
             const int per = nb / 8, rem = nb % 8, xcd = bid % 8;
             bid = xcd * per + (xcd < rem ? xcd : rem) + bid / 8;
         }
-        const int x0 = (bid % tiles_x) * TX;
-        const int y0 = ((bid / tiles_x) % tiles_y) * TY;
-        const int z0 = (bid / (tiles_x * tiles_y)) * TZ;
+        // z fastest: consecutive blocks of one XCD are z neighbours, whose regions share 2H of
+        // their TZ + 2H planes (3 % faster than x fastest on tensor192)
+        const int tiles_z = (a.nz + TZ - 1) / TZ;
+        const int z0 = (bid % tiles_z) * TZ;
+        const int x0 = ((bid / tiles_z) % tiles_x) * TX;
+        const int y0 = (bid / (tiles_z * tiles_x)) * TY;
         const int i = x0 + tx, j = y0 + ty, k = z0 + tz;
         const bool active = i < a.nx && j < a.ny && k < a.nz;
         const int off = active ? i + j * a.sj + k * a.sk : 0;
