@@ -100,7 +100,9 @@ typedef struct tsp_kernel_cfg {
                            or 32 for 32 x 32 tiles (two rows per wave) */
     int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
     int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
-    int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together) */
+    int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together);
+                           2: as 1, and RK4 stages 1 and 3 reverse every direction (a stage
+                           starts at the planes the previous stage wrote last) */
     int32_t halo_ring;  /* 1: tile_y 8/16 load each plane's x/y halo with its z-queue head and keep
                            it in LDS until use (falls back to 0 where the LDS does not fit) */
     int32_t stage_pairs; /* 1: RK4 stages (0,1) and (2,3) each run as one launch on 32 x 32 tiles

@@ -101,6 +101,8 @@ TILE_CFGS = [
     {"ghost_mode": 0, "tile_y": 8, "zchunk": 2, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
     {"ghost_mode": 0, "tile_y": 32, "zchunk": 6, "xcd_remap": 1, "zalt": 1, "halo_ring": 1},
     {"ghost_mode": 1, "tile_y": 32, "zchunk": 23, "xcd_remap": 0, "zalt": 0, "halo_ring": 0},
+    {"ghost_mode": 0, "tile_y": 16, "zchunk": 7, "xcd_remap": 1, "zalt": 2, "halo_ring": 1},
+    {"ghost_mode": 1, "tile_y": 8, "zchunk": 4, "xcd_remap": 0, "zalt": 2, "halo_ring": 0},
 ]
 
 
@@ -281,7 +283,7 @@ def _random_case(rng):
     per = tuple(int(v) for v in rng.integers(0, 2, 3))
     abl = int(rng.integers(1, min(n) // 2)) if min(n) >= 6 and rng.random() < 0.5 else 0
     cfg = {"ghost_mode": int(rng.integers(2)), "tile_y": int(rng.choice([2, 4, 8, 16, 32])),
-           "zchunk": int(rng.integers(1, n[2] + 1)), "xcd_remap": int(rng.integers(2)), "zalt": int(rng.integers(2)),
+           "zchunk": int(rng.integers(1, n[2] + 1)), "xcd_remap": int(rng.integers(2)), "zalt": int(rng.integers(3)),
            "halo_ring": int(rng.integers(2))}
     return system, n, sn, ko, per, abl, cfg
 

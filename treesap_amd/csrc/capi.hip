@@ -95,7 +95,7 @@ tsp_status validate_cfg(const tsp_kernel_cfg *c) {
         return fail(TSP_INVALID_ARG, "tile_y must be 2, 4, 8, 16 or 32");
     if (c->zchunk < 1) return fail(TSP_INVALID_ARG, "zchunk must be >= 1");
     if (c->xcd_remap != 0 && c->xcd_remap != 1) return fail(TSP_INVALID_ARG, "xcd_remap must be 0 or 1");
-    if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
+    if (c->zalt < 0 || c->zalt > 2) return fail(TSP_INVALID_ARG, "zalt must be 0, 1 or 2");
     if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
     if (c->stage_pairs != 0 && c->stage_pairs != 1) return fail(TSP_INVALID_ARG, "stage_pairs must be 0 or 1");
     return TSP_OK;
@@ -997,7 +997,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     for (int ty : {2, 4, 8, 16, 32})
         for (int zc : zchunks)
             for (int xr = 0; xr < 2; ++xr)
-                for (int za = 0; za < 2; ++za)
+                for (int za = 0; za < 3; ++za)
                     for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr, 0});
     // stage pairs (32 x 32 tiles; tile_y, zalt and halo_ring do not apply)
     if (pairs_apply(s, 0, nullptr))
