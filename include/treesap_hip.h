@@ -281,7 +281,8 @@ typedef struct tsp_batch_cfg {
     int32_t tile_y;      /* 8 or 16 rows per workgroup (tile_x 64) */
     int32_t zchunk;      /* z planes per work item; 0 = min(nz, 32) per instance */
     int32_t grid_blocks; /* persistent grid size; 0 = one workgroup per work item */
-    int32_t zalt;        /* 1: odd z chunks march downwards */
+    int32_t zalt;        /* 1: odd z chunks march downwards; 2: as 1, and forward RK4 stages 1 and 3
+                            reverse every direction (as tsp_kernel_cfg.zalt) */
     int32_t halo_ring;   /* 1: halo-ring tile body where its LDS fits (as tsp_kernel_cfg.halo_ring) */
     int32_t narrow_tiles; /* 1: an x remainder of <= 32 cells runs as 32 x 2*tile_y tiles (no idle lanes) */
 } tsp_batch_cfg;

@@ -43,6 +43,7 @@ BATCH_CFGS = [
     {"tile_y": 16, "zchunk": 64, "grid_blocks": 300, "zalt": 1, "halo_ring": 0, "narrow_tiles": 1},
     {"tile_y": 8, "zchunk": 0, "grid_blocks": 0, "zalt": 1, "halo_ring": 0, "narrow_tiles": 0},
     {"tile_y": 16, "zchunk": 3, "grid_blocks": 0, "zalt": 1, "halo_ring": 1, "narrow_tiles": 0},
+    {"tile_y": 8, "zchunk": 4, "grid_blocks": 0, "zalt": 2, "halo_ring": 1, "narrow_tiles": 1},
 ]
 
 
@@ -144,7 +145,7 @@ def test_random_batches_match_oracle(seed):
         cases.append((case.replace(dt=case.dt * (1.0 + 0.05 * k)), state))
     # tile height, halo ring and narrow tiles cover their combinations over the seeds; the rest is random
     cfg = {"tile_y": [8, 16][seed % 2], "zchunk": int(rng.choice([0, 1, 5, 64])),
-           "grid_blocks": int(rng.choice([0, 7, 300])), "zalt": int(rng.integers(2)),
+           "grid_blocks": int(rng.choice([0, 7, 300])), "zalt": int(rng.integers(3)),
            "halo_ring": [1, 1, 0, 1, 0, 1][seed], "narrow_tiles": [1, 1, 1, 0, 0, 1][seed]}
     _say(f"[batch] random seed={seed} {len(cases)} instances {[c.n for c, _ in cases]} cfg={cfg}")
     b = Batch([c for c, _ in cases])

@@ -91,7 +91,7 @@ tsp_status validate_batch_cfg(const tsp_batch_cfg *c) {
     if (c->tile_y != 8 && c->tile_y != 16) return fail(TSP_INVALID_ARG, "batch tile_y must be 8 or 16");
     if (c->zchunk < 0) return fail(TSP_INVALID_ARG, "zchunk must be >= 0 (0: automatic)");
     if (c->grid_blocks < 0) return fail(TSP_INVALID_ARG, "grid_blocks must be >= 0 (0: one block per work item)");
-    if (c->zalt != 0 && c->zalt != 1) return fail(TSP_INVALID_ARG, "zalt must be 0 or 1");
+    if (c->zalt < 0 || c->zalt > 2) return fail(TSP_INVALID_ARG, "zalt must be 0, 1 or 2");
     if (c->halo_ring != 0 && c->halo_ring != 1) return fail(TSP_INVALID_ARG, "halo_ring must be 0 or 1");
     if (c->narrow_tiles != 0 && c->narrow_tiles != 1) return fail(TSP_INVALID_ARG, "narrow_tiles must be 0 or 1");
     return TSP_OK;
@@ -720,7 +720,8 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
         e = copy_old(false);
         return e == hipSuccess ? TSP_OK : hip_fail(e, "batch autotune restore");
     };
-    // every candidate over `steps` steps, then the three fastest over three times as many
+    // every candidate over `steps` steps, then the three fastest over three times as
+    // many, each also with zalt 2 (stages 1 and 3 reverse the march)
     std::vector<std::pair<double, int>> ranked;
     for (size_t i = 0; i < cands.size() && st == TSP_OK; ++i) {
         double ms = 0.0;
@@ -730,14 +731,18 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
     double best_ms = 1e300;
     if (st == TSP_OK) {
         std::sort(ranked.begin(), ranked.end());
-        for (size_t q = 0; q < std::min<size_t>(3, ranked.size()) && st == TSP_OK; ++q) {
-            double ms = 0.0;
-            st = time_cfg(cands[ranked[q].second], 3 * steps, &ms);
-            if (st == TSP_OK && ms < best_ms) {
-                best_ms = ms;
-                best = cands[ranked[q].second];
+        for (size_t q = 0; q < std::min<size_t>(3, ranked.size()) && st == TSP_OK; ++q)
+            for (int za : {1, 2}) {
+                tsp_batch_cfg c = cands[ranked[q].second];
+                c.zalt = za;
+                double ms = 0.0;
+                st = time_cfg(c, 3 * steps, &ms);
+                if (st != TSP_OK) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
             }
-        }
     }
     // restore (the state is already restored after every candidate)
     std::memcpy(b->role, saved_role, sizeof(saved_role));

@@ -161,7 +161,9 @@ template <class Sys, int N, int R, int TY, int RS, int SM = -1, bool ADJ = false
             const int z1 = min(z0 + I.zchunk, I.nz);
             __syncthreads(); // the previous item's waves may still read LDS
             const double *sig = FS::stage_sigma(I.sigma_tab, I.abl_T, sig_lds);
-            const bool down = b.zalt && (wk.zc & 1);
+            // zalt 2: forward stages 1 and 3 also reverse every direction (as the single-box kernel)
+            const bool flip = !ADJ && b.zalt == 2 && (b.stage == kStage1 || b.stage == kStage3);
+            const bool down = (b.zalt != 0 && (wk.zc & 1)) != flip;
             if (wk.narrow) {
                 if (down) NFS::template run_tile<true>(a, wk.x0, wk.y0, z0, z1, lds, sig);
                 else NFS::template run_tile<false>(a, wk.x0, wk.y0, z0, z1, lds, sig);
