@@ -35,6 +35,10 @@ def test_struct_layout_matches_c(tmp_path):
             printf("desc %zu\\n", sizeof(tsp_desc));
             printf("cfg %zu\\n", sizeof(tsp_kernel_cfg));
             printf("bcfg %zu\\n", sizeof(tsp_batch_cfg));
+            printf("expr %zu\\n", sizeof(tsp_expr_op));
+            printf("expr_value %zu\\n", offsetof(tsp_expr_op, value));
+            printf("heaviside %d\\n", (int)TSP_EXPR_HEAVISIDE);
+            printf("maxstack %d\\n", (int)TSP_EXPR_MAX_STACK);
     """)
     for f in fields:
         prog += f'    printf("{f} %zu\\n", offsetof(tsp_desc, {f}));\n'
@@ -48,6 +52,8 @@ def test_struct_layout_matches_c(tmp_path):
     assert int(out["desc"]) == C.sizeof(_lib.TspDesc)
     assert int(out["cfg"]) == C.sizeof(_lib.TspKernelCfg)
     assert int(out["bcfg"]) == C.sizeof(_lib.TspBatchCfg)
+    assert int(out["expr"]) == C.sizeof(_lib.TspExprOp) and int(out["expr_value"]) == _lib.TspExprOp.value.offset
+    assert int(out["heaviside"]) == _lib.EXPR_CODES["heaviside"] and int(out["maxstack"]) == _lib.TSP_EXPR_MAX_STACK
     for f in fields:
         assert int(out[f]) == getattr(_lib.TspDesc, f).offset, f
 
@@ -106,6 +112,15 @@ def test_null_handles_are_rejected():
     assert L.tsp_sync(None) == _lib.INVALID_ARG
     assert L.tsp_destroy(None) == _lib.OK
     assert L.tsp_create(None, None, None) == _lib.INVALID_ARG
+    assert L.tsp_init_expr(None, None, None, None) == _lib.INVALID_ARG
+    assert L.tsp_level_init_expr(None, None, None, None) == _lib.INVALID_ARG
+    assert L.tsp_get_stream(None, None) == _lib.INVALID_ARG
+    assert L.tsp_batch_get_stream(None, None) == _lib.INVALID_ARG
+    assert L.tsp_timer_create(None, None) == _lib.INVALID_ARG
+    assert L.tsp_timer_start(None) == _lib.INVALID_ARG
+    assert L.tsp_timer_stop(None, None) == _lib.INVALID_ARG
+    assert L.tsp_timer_destroy(None) == _lib.OK
+    assert L.tsp_rk4_steps_noalloc(None, None, None, 1) == _lib.INVALID_ARG
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
