@@ -749,8 +749,13 @@ tsp_status tsp_timer_create(void *stream, tsp_timer **out) {
     tsp_timer *t = new (std::nothrow) tsp_timer;
     if (!t) return fail(TSP_OOM, "host allocation failed");
     t->stream = (hipStream_t)stream;
-    hipError_t e = hipEventCreate(&t->start);
-    if (e == hipSuccess) e = hipEventCreate(&t->stop);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) {
+        t->start = ev[0];
+        e = hipEventCreate(&ev[1]);
+        if (e == hipSuccess) t->stop = ev[1];
+    }
     if (e != hipSuccess) {
         tsp_timer_destroy(t);
         return fail(TSP_HIP_ERROR, std::string("hipEventCreate: ") + hipGetErrorString(e));
