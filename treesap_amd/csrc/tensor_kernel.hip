@@ -40,6 +40,7 @@ __device__ __forceinline__ double pick3(int m, double v0, double v1, double v2) 
 
 template <int N, int R> struct TensorTile {
     static constexpr int TX = 16, TY = 8, TZ = 2, NT = TX * TY * TZ; // 256 threads: two workgroups per CU
+    static constexpr int ZG = 2; // z tiles per slab of the block order
     static constexpr int H = N > R ? N : R;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, REG = RX * RY * RZ;
 
@@ -54,7 +55,10 @@ template <int N, int R> struct TensorTile {
     static constexpr int P1 = ssize<1> + 2 * ssize<0>;
     static constexpr int P2 = 2 * ssize<2> + ssize<1>;
     static constexpr int GBUF = P1 > P2 ? P1 : P2;
-    static constexpr int LDS_DOUBLES = REG + GBUF;
+    // with KO (R > 0): a second region buffer for the K component whose KO
+    // term is evaluated in the same pass, and six KO terms per thread
+    static constexpr int KBUF = R > 0 ? REG + 6 * NT : 0;
+    static constexpr int LDS_DOUBLES = REG + GBUF + KBUF;
 
     static __device__ __forceinline__ int ridx(int rx, int ry, int rz) { return (rz * RY + ry) * RX + rx; }
 
@@ -132,6 +136,8 @@ template <int N, int R> struct TensorTile {
     static __device__ void run(const StageArgs &a, double *lds) {
         double *u = lds;
         double *g = lds + REG;
+        double *v = g + GBUF;   // K component region (R > 0)
+        double *kob = v + REG;  // KO terms of the K components, [c - 6][tid] (R > 0)
         const int tid = threadIdx.x;
         const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
         const int tiles_x = (a.nx + TX - 1) / TX;
@@ -143,12 +149,25 @@ template <int N, int R> struct TensorTile {
             const int per = nb / 8, rem = nb % 8, xcd = bid % 8;
             bid = xcd * per + (xcd < rem ? xcd : rem) + bid / 8;
         }
-        // z fastest: consecutive blocks of one XCD are z neighbours, whose regions share 2H of
-        // their TZ + 2H planes (3 % faster than x fastest on tensor192)
+        // Each XCD's blocks (a contiguous range of bid after the remap) cover one
+        // group of GS consecutive (x, y) tiles (x fastest) over all z, swept in
+        // slabs of ZG z tiles: a slab's GS * ZG blocks, about what the XCD's
+        // CUs hold at once, share their halos in the XCD's L2, and the next
+        // slab rereads the previous one's last planes from there.
+        const int tiles_y = (a.ny + TY - 1) / TY;
         const int tiles_z = (a.nz + TZ - 1) / TZ;
-        const int z0 = (bid % tiles_z) * TZ;
-        const int x0 = ((bid / tiles_z) % tiles_x) * TX;
-        const int y0 = (bid / (tiles_z * tiles_x)) * TY;
+        const int nxy = tiles_x * tiles_y;
+        const int GS = (nxy + 7) / 8;
+        const int grp = bid / (GS * tiles_z);                 // group of (x, y) tiles
+        const int r = bid - grp * GS * tiles_z;               // index inside the group
+        const int gsz = min(GS, nxy - grp * GS);              // tiles in this group (the last may hold fewer)
+        const int slab = r / (gsz * ZG);
+        const int r2 = r - slab * gsz * ZG;
+        const int zn = min(ZG, tiles_z - slab * ZG);          // z tiles in this slab (the last may hold fewer)
+        const int txy = grp * GS + r2 / zn;
+        const int z0 = (slab * ZG + r2 % zn) * TZ;
+        const int x0 = (txy % tiles_x) * TX;
+        const int y0 = (txy / tiles_x) * TY;
         const int i = x0 + tx, j = y0 + ty, k = z0 + tz;
         const bool active = i < a.nx && j < a.ny && k < a.nz;
         const int off = active ? i + j * a.sj + k * a.sk : 0;
@@ -174,8 +193,13 @@ template <int N, int R> struct TensorTile {
         // loaded into registers one component ahead and written to LDS at the
         // start of that component's pass, so its latency overlaps the previous
         // component's arithmetic.
+        // Not every region point is read: no stencil reaches a corner of the
+        // region (outside the block along all three axes), and the K
+        // components' KO stencils read only its star (outside along at most
+        // one axis). The point sets are bit masks over the thread's points.
         constexpr int RPT = (REG + NT - 1) / NT;
         int roff[RPT];
+        unsigned hmask = 0, kmask = 0;
 #pragma unroll
         for (int m = 0; m < RPT; ++m) {
             const int p = min(tid + m * NT, REG - 1);
@@ -184,17 +208,25 @@ template <int N, int R> struct TensorTile {
             const int y = min(y0 + ry - H, a.ny + a.G - 1);
             const int z = min(z0 + rz - H, a.nz + a.G - 1);
             roff[m] = x + y * a.sj + z * a.sk;
+            const int outside = (rx < H || rx >= H + TX) + (ry < H || ry >= H + TY) + (rz < H || rz >= H + TZ);
+            if (tid + m * NT < REG) {
+                if (outside < 3) hmask |= 1u << m;
+                if (outside < 2) kmask |= 1u << m;
+            }
         }
-        double pre[RPT];
-        auto load_region = [&](int c) {
+        double pre[RPT], prek[RPT];
+        auto load_region = [&](int c, double (&dst)[RPT]) {
             const double *src = a.src + (long long)c * a.sc;
-#pragma unroll
-            for (int m = 0; m < RPT; ++m) pre[m] = src[roff[m]];
-        };
-        auto stage_region = [&]() {
+            const unsigned mask = c < 6 ? hmask : kmask;
 #pragma unroll
             for (int m = 0; m < RPT; ++m)
-                if (tid + m * NT < REG) u[tid + m * NT] = pre[m];
+                if (mask >> m & 1u) dst[m] = src[roff[m]];
+        };
+        auto stage_region = [&](int c, const double (&from)[RPT], double *buf) {
+            const unsigned mask = c < 6 ? hmask : kmask;
+#pragma unroll
+            for (int m = 0; m < RPT; ++m)
+                if (mask >> m & 1u) buf[tid + m * NT] = from[m];
         };
         // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154)
         auto update = [&](int c, double kv, double cenc, double initv, double accv) {
@@ -209,12 +241,12 @@ template <int N, int R> struct TensorTile {
             if (T > 0) kv = kv - sigma * (cenc - a.target[c]);
             return kv;
         };
-        auto ko_sum = [&]() {
+        auto ko_sum = [&](const double *w) {
             double t = 0.0;
             if constexpr (R > 0) {
-                t += ko<0>(u, own, a);
-                t += ko<1>(u, own, a);
-                t += ko<2>(u, own, a);
+                t += ko<0>(w, own, a);
+                t += ko<1>(w, own, a);
+                t += ko<2>(w, own, a);
             }
             return t;
         };
@@ -229,12 +261,18 @@ template <int N, int R> struct TensorTile {
             });
         });
 
-        // h components in increasing order (the reference's contraction order)
-        load_region(0);
+        // h components in increasing order (the reference's contraction
+        // order). With KO, pass c also stages K component c's region and
+        // evaluates its KO term (kept in LDS until the K updates), so the K
+        // components need no passes of their own: a pass with little
+        // arithmetic would wait out the latency of every region load.
+        load_region(0, pre);
+        if constexpr (R > 0) load_region(6, prek);
 #pragma unroll 1
         for (int c = 0; c < 6; ++c) {
-            __syncthreads(); // the previous component's reads of u and g are done
-            stage_region();
+            __syncthreads(); // the previous component's reads of u, v and g are done
+            stage_region(c, pre, u);
+            if constexpr (R > 0) stage_region(6 + c, prek, v);
             // centre values first, then the prefetch: the memory counter
             // completes in order, so the update's wait for the centre values
             // then leaves the next region's loads in flight
@@ -244,39 +282,54 @@ template <int N, int R> struct TensorTile {
                 if (read_init) initv = a.init[off + (long long)c * a.sc];
                 if (!is_first) accv = a.acc[off + (long long)c * a.sc];
             }
-            if (c < 5 || R > 0) load_region(c + 1); // next h component, or the first K component for KO
+            if (c < 5) {
+                load_region(c + 1, pre);
+                if constexpr (R > 0) load_region(7 + c, prek);
+            }
             __syncthreads();
 
             double dv[3][3];
             dv[0][0] = d2<0>(u, own, a);
             dv[1][1] = d2<1>(u, own, a);
             dv[2][2] = d2<2>(u, own, a);
-            const double ko_term = ko_sum();
+            const double ko_term = ko_sum(u);
             const double cenc = u[own];
+            if constexpr (R > 0) kob[c * NT + tid] = ko_sum(v); // read back by this thread only
+
+            const int row = c >= 3 ? 2 : c >= 1 ? 1 : 0;
+            const int col = c - (row * (row + 1)) / 2;
+            // Mixed derivatives this component feeds (outer p, inner q): f takes
+            // D_row D_q and D_col D_q for every q, tdd takes D_p D_q (q < p) of
+            // the diagonal components. The others are not evaluated:
+            // D_y D_x (c != 3), D_x D_y and D_x D_z (c in {0, 1, 3}),
+            // D_z D_x and D_z D_y (c != 1), D_y D_z (c in {1, 2, 4}).
+            const bool need_yx = c != 3, need_xy = c <= 1 || c == 3, need_zx = c != 1;
+            const bool need_yz = c == 1 || c == 2 || c == 4;
 
             // phase 1 slabs: g_x along y, g_y along x, g_z along x
             constexpr int SA = ssize<1>, SB = ssize<0>;
-            fill_slab<0, 1>(u, g, tx, ty, tz, own, a);
-            fill_slab<1, 0>(u, g + SA, tx, ty, tz, own, a);
-            fill_slab<2, 0>(u, g + SA + SB, tx, ty, tz, own, a);
+            if (need_yx) fill_slab<0, 1>(u, g, tx, ty, tz, own, a);
+            if (need_xy) {
+                fill_slab<1, 0>(u, g + SA, tx, ty, tz, own, a);
+                fill_slab<2, 0>(u, g + SA + SB, tx, ty, tz, own, a);
+            }
             __syncthreads();
-            dv[1][0] = outer<1>(g, tx, ty, tz, a);
-            dv[0][1] = outer<0>(g + SA, tx, ty, tz, a);
-            dv[0][2] = outer<0>(g + SA + SB, tx, ty, tz, a);
+            dv[1][0] = need_yx ? outer<1>(g, tx, ty, tz, a) : 0.0;
+            dv[0][1] = need_xy ? outer<0>(g + SA, tx, ty, tz, a) : 0.0;
+            dv[0][2] = need_xy ? outer<0>(g + SA + SB, tx, ty, tz, a) : 0.0;
             __syncthreads();
 
             // phase 2 slabs: g_x along z, g_y along z, g_z along y
             constexpr int SD = ssize<2>;
-            fill_slab<0, 2>(u, g, tx, ty, tz, own, a);
-            fill_slab<1, 2>(u, g + SD, tx, ty, tz, own, a);
-            fill_slab<2, 1>(u, g + 2 * SD, tx, ty, tz, own, a);
+            if (need_zx) {
+                fill_slab<0, 2>(u, g, tx, ty, tz, own, a);
+                fill_slab<1, 2>(u, g + SD, tx, ty, tz, own, a);
+            }
+            if (need_yz) fill_slab<2, 1>(u, g + 2 * SD, tx, ty, tz, own, a);
             __syncthreads();
-            dv[2][0] = outer<2>(g, tx, ty, tz, a);
-            dv[2][1] = outer<2>(g + SD, tx, ty, tz, a);
-            dv[1][2] = outer<1>(g + 2 * SD, tx, ty, tz, a);
-
-            const int row = c >= 3 ? 2 : c >= 1 ? 1 : 0;
-            const int col = c - (row * (row + 1)) / 2;
+            dv[2][0] = need_zx ? outer<2>(g, tx, ty, tz, a) : 0.0;
+            dv[2][1] = need_zx ? outer<2>(g + SD, tx, ty, tz, a) : 0.0;
+            dv[1][2] = need_yz ? outer<1>(g + 2 * SD, tx, ty, tz, a) : 0.0;
             // f[A][B] gets D_m D_A h_{mB} where sym2(m, B) == c, i.e. (m, B) is (row, col) or (col, row)
             static_for<0, 3>([&]<int A>() {
                 const double by_row = pick3(row, dv[0][A], dv[1][A], dv[2][A]);
@@ -296,30 +349,17 @@ template <int N, int R> struct TensorTile {
             if (active) update(c, decorate(c, -2.0 * kc, ko_term, cenc), cenc, initv, accv);
         }
 
-        // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap)
-#pragma unroll 1
+        // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap); their centre,
+        // init and acc values are loaded together (unrolled loop)
+#pragma unroll
         for (int c = 6; c < 12; ++c) {
             double ko_term = 0.0, cenc = 0.0, initv = 0.0, accv = 0.0;
-            auto load_centre = [&]() {
-                if (active) {
-                    if (read_init) initv = a.init[off + (long long)c * a.sc];
-                    if (!is_first) accv = a.acc[off + (long long)c * a.sc];
-                }
-            };
-            if constexpr (R > 0) {
-                __syncthreads();
-                stage_region();
-                load_centre(); // before the prefetch (see the h loop)
-                // unconditional (the last component rereads its own, cached
-                // region), so the wait counting stays exact through the loop
-                load_region(min(c + 1, 11));
-                __syncthreads();
-                ko_term = ko_sum();
-                cenc = u[own];
-            } else {
-                if (active) cenc = a.src[off + (long long)c * a.sc];
-                load_centre();
+            if (active) {
+                cenc = a.src[off + (long long)c * a.sc];
+                if (read_init) initv = a.init[off + (long long)c * a.sc];
+                if (!is_first) accv = a.acc[off + (long long)c * a.sc];
             }
+            if constexpr (R > 0) ko_term = kob[(c - 6) * NT + tid];
             const int row = c >= 9 ? 2 : c >= 7 ? 1 : 0;
             const int col = (c - 6) - (row * (row + 1)) / 2;
             double fab = 0.0, fba = 0.0, tv = 0.0, lv = 0.0;
