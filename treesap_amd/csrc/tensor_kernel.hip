@@ -50,11 +50,12 @@ template <int N, int R> struct TensorTile {
     template <int EA> static constexpr int ez = TZ + (EA == 2 ? 2 * N : 0);
     template <int EA> static constexpr int ssize = ex<EA> * ey<EA> * ez<EA>;
 
-    // phase 1: g_x along y (D_y D_x), g_y along x (D_x D_y), g_z along x (D_x D_z)
-    // phase 2: g_x along z (D_z D_x), g_y along z (D_z D_y), g_z along y (D_y D_z)
+    // slabs: g_x along y (D_y D_x), g_y along x (D_x D_y), g_z along x (D_x D_z),
+    // g_x along z (D_z D_x), g_y along z (D_z D_y), g_z along y (D_y D_z), each
+    // with its own LDS, so one barrier separates all fills from all outer derivatives
     static constexpr int P1 = ssize<1> + 2 * ssize<0>;
     static constexpr int P2 = 2 * ssize<2> + ssize<1>;
-    static constexpr int GBUF = P1 > P2 ? P1 : P2;
+    static constexpr int GBUF = P1 + P2;
     // with KO (R > 0): a second region buffer for the K component whose KO
     // term is evaluated in the same pass, and six KO terms per thread
     static constexpr int KBUF = R > 0 ? REG + 6 * NT : 0;
@@ -306,31 +307,27 @@ template <int N, int R> struct TensorTile {
             const bool need_yx = c != 3, need_xy = c <= 1 || c == 3, need_zx = c != 1;
             const bool need_yz = c == 1 || c == 2 || c == 4;
 
-            // phase 1 slabs: g_x along y, g_y along x, g_z along x
-            constexpr int SA = ssize<1>, SB = ssize<0>;
+            // slabs of g_x along y, g_y along x, g_z along x, g_x along z, g_y along z, g_z along y
+            constexpr int SA = ssize<1>, SB = ssize<0>, SD = ssize<2>;
+            double *g2 = g + P1;
             if (need_yx) fill_slab<0, 1>(u, g, tx, ty, tz, own, a);
             if (need_xy) {
                 fill_slab<1, 0>(u, g + SA, tx, ty, tz, own, a);
                 fill_slab<2, 0>(u, g + SA + SB, tx, ty, tz, own, a);
             }
+            if (need_zx) {
+                fill_slab<0, 2>(u, g2, tx, ty, tz, own, a);
+                fill_slab<1, 2>(u, g2 + SD, tx, ty, tz, own, a);
+            }
+            if (need_yz) fill_slab<2, 1>(u, g2 + 2 * SD, tx, ty, tz, own, a);
             __syncthreads();
             dv[1][0] = need_yx ? outer<1>(g, tx, ty, tz, a) : 0.0;
             dv[0][1] = need_xy ? outer<0>(g + SA, tx, ty, tz, a) : 0.0;
             dv[0][2] = need_xy ? outer<0>(g + SA + SB, tx, ty, tz, a) : 0.0;
-            __syncthreads();
+            dv[2][0] = need_zx ? outer<2>(g2, tx, ty, tz, a) : 0.0;
+            dv[2][1] = need_zx ? outer<2>(g2 + SD, tx, ty, tz, a) : 0.0;
+            dv[1][2] = need_yz ? outer<1>(g2 + 2 * SD, tx, ty, tz, a) : 0.0;
 
-            // phase 2 slabs: g_x along z, g_y along z, g_z along y
-            constexpr int SD = ssize<2>;
-            if (need_zx) {
-                fill_slab<0, 2>(u, g, tx, ty, tz, own, a);
-                fill_slab<1, 2>(u, g + SD, tx, ty, tz, own, a);
-            }
-            if (need_yz) fill_slab<2, 1>(u, g + 2 * SD, tx, ty, tz, own, a);
-            __syncthreads();
-            dv[2][0] = need_zx ? outer<2>(g, tx, ty, tz, a) : 0.0;
-            dv[2][1] = need_zx ? outer<2>(g + SD, tx, ty, tz, a) : 0.0;
-            dv[1][2] = need_yz ? outer<1>(g + 2 * SD, tx, ty, tz, a) : 0.0;
-            // f[A][B] gets D_m D_A h_{mB} where sym2(m, B) == c, i.e. (m, B) is (row, col) or (col, row)
             static_for<0, 3>([&]<int A>() {
                 const double by_row = pick3(row, dv[0][A], dv[1][A], dv[2][A]);
                 const double by_col = pick3(col, dv[0][A], dv[1][A], dv[2][A]);
