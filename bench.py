@@ -104,7 +104,8 @@ def cpu_baseline(case, state, budget_s: float):
 def ensemble_leg(count: int, steps: int, tune: bool):
     """Batched vs plain on the ensemble workload: `count` periodic (phi, Pi) instances (side in
     {32,48,64,96,128}, n in {2,3,4}, KO order n), `steps` RK4 steps each.
-      batched: one Batch, graph-replayed steps, one persistent launch per stage per stencil group;
+      batched: one Batch, graph-replayed steps, one persistent launch per stage per stencil group,
+               each group with the configuration the library's autotuner chose for it;
       plain:   the instances one after another, each with the single-box solver and eager per-step
                launches, summed device time."""
     import torch
@@ -121,6 +122,9 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     # the library's batch autotuner (tsp_batch_autotune): every candidate configuration timed on
     # this batch, the fastest kept; the state is restored
     best = (None, b.autotune(6) if tune else b.kernel)
+    # every stencil-width group has its own configuration
+    groups = sorted({(c.stencil_n, c.ko_order) for c, _ in params})
+    group_cfgs = {f"n{sn}_ko{ko}": b.group_kernel(sn, ko) for sn, ko in groups}
     b.graph_rk4_steps(2)  # capture the graph of the chosen configuration (an even count keeps the parity)
     for i, (case, centre) in enumerate(params):
         b.set_state(i, gaussian_state_device(case, centre))
@@ -157,6 +161,7 @@ def ensemble_leg(count: int, steps: int, tune: bool):
         "batched_s": batched_ms / 1e3,
         "plain_s": plain_ms / 1e3,
         "batch_kernel": best[1],
+        "batch_kernel_per_group": group_cfgs,
         "batch_kernel_chosen_by": "tsp_batch_autotune" if tune else "default",
         "batched_equals_plain": "bitwise, every instance",
     }

@@ -298,8 +298,19 @@ tsp_status tsp_batch_size(tsp_batch *b, int32_t *count, int64_t *cells);
 tsp_status tsp_batch_set_state(tsp_batch *b, int32_t index, const double *src, int32_t on_device);
 tsp_status tsp_batch_get_state(tsp_batch *b, int32_t index, double *dst, int32_t on_device);
 tsp_status tsp_batch_get_time(tsp_batch *b, int32_t index, double *time, int64_t *steps);
+/* Kernel configuration: every (system, stencil width, KO order) group has its
+ * own (its launches and work items follow it). tsp_batch_set_kernel gives all
+ * groups one configuration; tsp_batch_get_kernel reports the last one set
+ * that way, or after tsp_batch_autotune the largest group's. The group
+ * variants address one group, by system and (stencil_n, ko_order);
+ * TSP_INVALID_ARG for a pair the batched kernels do not carry. Results never
+ * depend on the configuration. */
 tsp_status tsp_batch_get_kernel(tsp_batch *b, tsp_batch_cfg *cfg);
 tsp_status tsp_batch_set_kernel(tsp_batch *b, const tsp_batch_cfg *cfg);
+tsp_status tsp_batch_get_group_kernel(tsp_batch *b, int32_t system, int32_t stencil_n, int32_t ko_order,
+                                      tsp_batch_cfg *cfg);
+tsp_status tsp_batch_set_group_kernel(tsp_batch *b, int32_t system, int32_t stencil_n, int32_t ko_order,
+                                      const tsp_batch_cfg *cfg);
 /* `nsteps` RK4 steps of every instance: eager launches, or graph replay (two
  * steps per graph, odd remainder eager). */
 tsp_status tsp_batch_rk4_steps(tsp_batch *b, int64_t nsteps);
@@ -309,13 +320,16 @@ tsp_status tsp_batch_graph_rk4_steps(tsp_batch *b, int64_t nsteps);
 tsp_status tsp_batch_graph_prepare(tsp_batch *b);
 /* `nsteps` forward Euler steps of every instance (SingleLevelEulerIntegrator.hpp:66-88). */
 tsp_status tsp_batch_euler_steps(tsp_batch *b, int64_t nsteps);
-/* Time candidate batch configurations (tile height, z chunk, persistent grid,
- * halo ring, narrow tiles) over `steps` graph-replayed RK4 steps each, the
- * three fastest again over three times as many, and keep the fastest; every
- * instance's state, the register roles and the clock are restored. Results
- * are cached per (device arch, batch or level, instance count, cells, stencil
- * width and KO order groups) in the file named by TSP_TUNE_CACHE, if set. The
- * chosen configuration is written to *chosen if non-NULL. */
+/* Tune every group on its own, since its stencil width and KO order decide
+ * its best tile shape and LDS blocking: candidate configurations (tile
+ * height, z chunk, persistent grid, halo ring, narrow tiles) over `steps`
+ * graph-replayed RK4 steps in which only that group's kernels run, the three
+ * fastest again over three times as many (also with zalt 2), and the fastest
+ * kept for the group. Every instance's state, the register roles and the
+ * clock are restored. Results are cached per group and (device arch, batch or
+ * level, instance count, cells, groups present) in the file named by
+ * TSP_TUNE_CACHE, if set. The largest group's configuration is written to
+ * *chosen if non-NULL. */
 tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen);
 
 /* A level split into boxes: replaces the BoxArray that

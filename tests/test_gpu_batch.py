@@ -185,6 +185,48 @@ def test_batch_autotune_restores_state_and_keeps_results():
         assert np.array_equal(b.get_state(i), o.get_state()), i
 
 
+def test_group_kernels_run_their_own_configurations():
+    """Every (system, stencil width, KO order) group runs its own kernel configuration
+    (tsp_batch_set_group_kernel): with other tile heights, z chunks, grids, rings and narrow
+    tiles per group, every instance stays bitwise equal to the oracle, eager and graph-replayed;
+    tsp_batch_set_kernel gives every group one configuration again; pairs without batched
+    kernels and invalid configurations are refused."""
+    from treesap_amd._lib import TspError
+
+    _say("[batch] per-group configurations")
+    cases = mixed_cases()
+    per_group = {
+        (4, 4): {"tile_y": 8, "zchunk": 7, "grid_blocks": 0, "zalt": 2, "halo_ring": 1, "narrow_tiles": 1},
+        (2, 3): {"tile_y": 16, "zchunk": 64, "grid_blocks": 5, "zalt": 0, "halo_ring": 0, "narrow_tiles": 0},
+        (3, 3): {"tile_y": 8, "zchunk": 3, "grid_blocks": 0, "zalt": 1, "halo_ring": 0, "narrow_tiles": 1},
+        (2, 2): {"tile_y": 16, "zchunk": 0, "grid_blocks": 64, "zalt": 2, "halo_ring": 1, "narrow_tiles": 1},
+    }
+    b = Batch([c for c, _ in cases])
+    default = b.kernel
+    for (sn, ko), cfg in per_group.items():
+        b.set_group_kernel(sn, ko, cfg)
+    for (sn, ko), cfg in per_group.items():
+        assert b.group_kernel(sn, ko) == cfg
+    assert b.group_kernel(1, 1) == default and b.kernel == default
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.rk4_steps(3)
+    b.graph_rk4_steps(2)
+    for i, (case, st) in enumerate(cases):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(5)
+        assert np.array_equal(b.get_state(i), o.get_state()), (i, case.stencil_n, case.ko_order)
+    b.kernel = BATCH_CFGS[3]
+    assert all(b.group_kernel(sn, ko) == BATCH_CFGS[3] for sn, ko in list(per_group) + [(1, 1)])
+    with pytest.raises(TspError):
+        b.group_kernel(3, 1)  # no batched kernels for KO order 1 with n = 3
+    with pytest.raises(TspError):
+        b.set_group_kernel(2, 2, {"tile_y": 4})
+    assert b.group_kernel(2, 2) == BATCH_CFGS[3]
+    b.close()
+
+
 def test_mixed_system_batch_matches_oracle():
     """(phi, Pi), (phi, u) and NonLinear instances in one batch: one launch per stage and
     (system, stencil width, KO order) group; every instance bitwise equal to its single-box

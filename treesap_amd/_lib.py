@@ -143,6 +143,8 @@ PROTOTYPES = {
     "tsp_batch_get_time": (C.c_int, [_VP, C.c_int32, _DP, _I64P]),
     "tsp_batch_get_kernel": (C.c_int, [_VP, C.POINTER(TspBatchCfg)]),
     "tsp_batch_set_kernel": (C.c_int, [_VP, C.POINTER(TspBatchCfg)]),
+    "tsp_batch_get_group_kernel": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(TspBatchCfg)]),
+    "tsp_batch_set_group_kernel": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(TspBatchCfg)]),
     "tsp_batch_rk4_steps": (C.c_int, [_VP, C.c_int64]),
     "tsp_batch_graph_rk4_steps": (C.c_int, [_VP, C.c_int64]),
     "tsp_batch_graph_prepare": (C.c_int, [_VP]),

@@ -41,6 +41,16 @@ hipError_t launch_batched_group(int group, const BatchArgs &b, int tile_y, int g
 
 using namespace tsp_host;
 
+// one kernel configuration per work-list group (system, stencil width, KO order)
+struct GroupCfgs {
+    tsp_batch_cfg g[tsp::kBatchGroups];
+    bool operator==(const GroupCfgs &o) const { return std::memcmp(g, o.g, sizeof(g)) == 0; }
+    bool operator!=(const GroupCfgs &o) const { return !(*this == o); }
+    void fill(const tsp_batch_cfg &c) {
+        for (auto &x : g) x = c;
+    }
+};
+
 struct tsp_batch {
     int count = 0;
     std::vector<tsp_desc> d;
@@ -59,15 +69,19 @@ struct tsp_batch {
     int role[4] = {0, 1, 2, 3};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    tsp_batch_cfg cfg{16, 0, 0, 1, 1, 1};
-    tsp_batch_cfg work_cfg{-1, -1, -1, -1, -1, -1}; // configuration the work list was built for
+    tsp_batch_cfg cfg{16, 0, 0, 1, 1, 1}; // what tsp_batch_get_kernel reports (the largest group's)
+    GroupCfgs gcfg;                        // the kernel configuration of every work-list group
+    GroupCfgs work_gcfg;                   // configurations the work list was built for
+    bool work_valid = false;
+    int only_group = -1;                   // >= 0: steps launch this group only (the autotuner)
     std::vector<double> time;
     int64_t steps = 0;
     // two-step graphs, each for the register roles it was captured with
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_role[2][4] = {};
     int graph_next = 0;
-    tsp_batch_cfg graph_cfg{-1, -1, -1, -1, -1, -1};
+    GroupCfgs graph_gcfg;                  // configurations the graphs were captured with
+    int graph_only = -2;                   // only_group the graphs were captured with (-2: none captured)
     uint64_t work_gen = 0; // bumped whenever the work list is rebuilt (graphs of other objects hold it)
     // a level (tsp_level_create): the instances are the boxes of one domain and
     // every stage starts with the ghost exchange between them
@@ -114,15 +128,16 @@ void drop_batch_graphs(tsp_batch *b) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
-    b->graph_cfg = tsp_batch_cfg{-1, -1, -1, -1, -1, -1};
+    b->graph_only = -2;
     b->graph_next = 0;
 }
 
-// (Re)build the device work list: instances in decreasing cell count, each
-// split into 64 x tile_y column tiles and z chunks; with narrow_tiles, an x
-// remainder of at most 32 cells becomes a strip of 32 x 2*tile_y tiles.
+// (Re)build the device work list: per group, instances in decreasing cell
+// count, each split into 64 x tile_y column tiles and z chunks of the group's
+// configuration; with narrow_tiles, an x remainder of at most 32 cells becomes
+// a strip of 32 x 2*tile_y tiles.
 tsp_status build_work(tsp_batch *b) {
-    if (b->d_work && std::memcmp(&b->work_cfg, &b->cfg, sizeof(tsp_batch_cfg)) == 0) return TSP_OK;
+    if (b->d_work && b->work_valid && b->work_gcfg == b->gcfg) return TSP_OK;
     std::vector<int> order(b->count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -131,17 +146,18 @@ tsp_status build_work(tsp_batch *b) {
         return cx > cy;
     });
     std::vector<tsp::BatchWork> work;
-    const int ty = b->cfg.tile_y;
     for (int v = 0; v < tsp::kBatchGroups; ++v) {
+        const tsp_batch_cfg &gc = b->gcfg.g[v];
+        const int ty = gc.tile_y;
         b->group_off[v] = (int)work.size();
         for (int i : order) {
             if (b->h_inst[i].variant != v) continue;
             const tsp_desc &d = b->d[i];
-            const int zc = b->cfg.zchunk > 0 ? std::min(b->cfg.zchunk, d.n[2]) : auto_zchunk(d.n[2]);
+            const int zc = gc.zchunk > 0 ? std::min(gc.zchunk, d.n[2]) : auto_zchunk(d.n[2]);
             b->h_inst[i].zchunk = zc;
             const int nzc = (d.n[2] + zc - 1) / zc;
             // x origin of the narrow strip (nx if there is none)
-            const int xn = b->cfg.narrow_tiles && d.n[0] % 64 != 0 && d.n[0] % 64 <= 32 ? d.n[0] / 64 * 64 : d.n[0];
+            const int xn = gc.narrow_tiles && d.n[0] % 64 != 0 && d.n[0] % 64 <= 32 ? d.n[0] / 64 * 64 : d.n[0];
             for (int c = 0; c < nzc; ++c) {
                 for (int y0 = 0; y0 < d.n[1]; y0 += ty)
                     for (int x0 = 0; x0 < xn; x0 += 64) work.push_back(tsp::BatchWork{i, x0, y0, c, 0});
@@ -163,7 +179,8 @@ tsp_status build_work(tsp_batch *b) {
     TSP_HIP(hipMemcpy(b->d_work, work.data(), sizeof(tsp::BatchWork) * work.size(), hipMemcpyHostToDevice));
     TSP_HIP(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(tsp::BatchInstance) * b->count, hipMemcpyHostToDevice));
     b->nwork = (int)work.size();
-    b->work_cfg = b->cfg;
+    b->work_gcfg = b->gcfg;
+    b->work_valid = true;
     b->work_gen++;
     return TSP_OK;
 }
@@ -178,7 +195,8 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
         if (b->level) // FillPatch of the stage's source register across the boxes
             TSP_HIP(tsp::launch_level_exchange(b->largs, stage == 0 ? OLD : ARG, b->max_shell, b->stream));
         for (int v = 0; v < tsp::kBatchGroups; ++v) {
-            if (b->group_n[v] == 0) continue;
+            if (b->group_n[v] == 0 || (b->only_group >= 0 && v != b->only_group)) continue;
+            const tsp_batch_cfg &gc = b->gcfg.g[v];
             tsp::BatchArgs a{};
             a.inst = b->d_inst;
             a.work = b->d_work + b->group_off[v];
@@ -188,10 +206,10 @@ tsp_status enqueue_batch_step(tsp_batch *b) {
             a.init = OLD;
             a.acc = NEW;
             a.argn = stage < 3 ? ARGN : -1;
-            a.zalt = b->cfg.zalt;
+            a.zalt = gc.zalt;
             a.ghost_mode = b->level ? 1 : 0; // a box's ghosts hold its neighbours' values after the exchange
-            const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-            TSP_HIP(tsp::launch_batched_group(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
+            const int grid = gc.grid_blocks > 0 ? std::min(gc.grid_blocks, a.nwork) : a.nwork;
+            TSP_HIP(tsp::launch_batched_group(v, a, gc.tile_y, grid, b->max_abl_T, gc.halo_ring, b->stream));
         }
     }
     std::swap(OLD, NEW);
@@ -203,7 +221,8 @@ tsp_status enqueue_batch_euler_step(tsp_batch *b) {
     int &OLD = b->role[TSP_ROLE_OLD], &NEW = b->role[TSP_ROLE_NEW];
     if (b->level) TSP_HIP(tsp::launch_level_exchange(b->largs, OLD, b->max_shell, b->stream));
     for (int v = 0; v < tsp::kBatchGroups; ++v) {
-        if (b->group_n[v] == 0) continue;
+        if (b->group_n[v] == 0 || (b->only_group >= 0 && v != b->only_group)) continue;
+        const tsp_batch_cfg &gc = b->gcfg.g[v];
         tsp::BatchArgs a{};
         a.inst = b->d_inst;
         a.work = b->d_work + b->group_off[v];
@@ -213,10 +232,10 @@ tsp_status enqueue_batch_euler_step(tsp_batch *b) {
         a.init = OLD;
         a.acc = NEW;
         a.argn = -1;
-        a.zalt = b->cfg.zalt;
+        a.zalt = gc.zalt;
         a.ghost_mode = b->level ? 1 : 0;
-        const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, a.nwork) : a.nwork;
-        TSP_HIP(tsp::launch_batched_group(v, a, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring, b->stream));
+        const int grid = gc.grid_blocks > 0 ? std::min(gc.grid_blocks, a.nwork) : a.nwork;
+        TSP_HIP(tsp::launch_batched_group(v, a, gc.tile_y, grid, b->max_abl_T, gc.halo_ring, b->stream));
     }
     std::swap(OLD, NEW);
     return TSP_OK;
@@ -232,9 +251,10 @@ void batch_advance_clock(tsp_batch *b) {
 // roles if missing; two RK4 steps bring the roles back); graphs of another
 // configuration are dropped first.
 tsp_status ensure_batch_graph(tsp_batch *b, const int roles[4], int *slot) {
-    if (std::memcmp(&b->graph_cfg, &b->cfg, sizeof(tsp_batch_cfg)) != 0) {
+    if (b->graph_only != b->only_group || b->graph_gcfg != b->gcfg) {
         drop_batch_graphs(b);
-        b->graph_cfg = b->cfg;
+        b->graph_gcfg = b->gcfg;
+        b->graph_only = b->only_group;
     }
     for (int g = 0; g < 2; ++g)
         if (b->graph[g] && std::memcmp(b->graph_role[g], roles, sizeof(b->graph_role[g])) == 0) {
@@ -310,6 +330,7 @@ tsp_status create_batch(const tsp_desc *descs, int32_t count, void *stream, tsp_
         if (st != TSP_OK) return st;
     }
     tsp_batch *b = new (std::nothrow) tsp_batch();
+    if (b) b->gcfg.fill(b->cfg);
     if (!b) return fail(TSP_OOM, "host allocation failed");
     auto cleanup = [&](tsp_status e) {
         tsp_batch_destroy(b);
@@ -508,6 +529,38 @@ tsp_status tsp_batch_set_kernel(tsp_batch *b, const tsp_batch_cfg *cfg) {
     tsp_status st = validate_batch_cfg(cfg);
     if (st != TSP_OK) return st;
     b->cfg = *cfg;
+    b->gcfg.fill(*cfg);
+    return build_work(b);
+}
+
+namespace {
+tsp_status group_of(tsp_batch *b, int32_t system, int32_t stencil_n, int32_t ko_order, int *v) {
+    if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
+    *v = tsp::batch_group(system, stencil_n, ko_order);
+    if (*v < 0)
+        return fail(TSP_INVALID_ARG, "no batched kernels for system " + std::to_string(system) + ", stencil_n " +
+                                         std::to_string(stencil_n) + ", ko_order " + std::to_string(ko_order));
+    return TSP_OK;
+}
+} // namespace
+
+tsp_status tsp_batch_get_group_kernel(tsp_batch *b, int32_t system, int32_t stencil_n, int32_t ko_order,
+                                      tsp_batch_cfg *cfg) {
+    int v = -1;
+    tsp_status st = group_of(b, system, stencil_n, ko_order, &v);
+    if (st != TSP_OK) return st;
+    if (!cfg) return fail(TSP_INVALID_ARG, "cfg is NULL");
+    *cfg = b->gcfg.g[v];
+    return TSP_OK;
+}
+
+tsp_status tsp_batch_set_group_kernel(tsp_batch *b, int32_t system, int32_t stencil_n, int32_t ko_order,
+                                      const tsp_batch_cfg *cfg) {
+    int v = -1;
+    tsp_status st = group_of(b, system, stencil_n, ko_order, &v);
+    if (st != TSP_OK) return st;
+    if ((st = validate_batch_cfg(cfg)) != TSP_OK) return st;
+    b->gcfg.g[v] = *cfg;
     return build_work(b);
 }
 
@@ -652,6 +705,10 @@ static void batch_cache_store(const std::string &key, const tsp_batch_cfg &c, do
     std::fclose(f);
 }
 
+// Tunes every work-list group on its own (steps launch that group only): its
+// stencil width and KO order set what its best tile shape, z chunk and LDS
+// blocking are. The largest group's choice is what tsp_batch_get_kernel
+// reports and *chosen receives.
 tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen) {
     if (!b) return fail(TSP_INVALID_ARG, "batch is NULL");
     if (steps < 2) steps = 2;
@@ -659,10 +716,24 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
     tsp_status st = build_work(b);
     if (st != TSP_OK) return st;
     const std::string key = batch_tune_key(b);
-    tsp_batch_cfg best = b->cfg;
-    if (batch_cache_lookup(key, &best)) {
-        b->cfg = best;
-        if (chosen) *chosen = best;
+    // groups present, most cells first
+    std::vector<std::pair<int64_t, int>> present;
+    for (int v = 0; v < tsp::kBatchGroups; ++v) {
+        if (b->group_n[v] == 0) continue;
+        int64_t cells = 0;
+        for (int i = 0; i < b->count; ++i)
+            if (b->h_inst[i].variant == v) cells += (int64_t)b->d[i].n[0] * b->d[i].n[1] * b->d[i].n[2];
+        present.push_back({-cells, v});
+    }
+    std::sort(present.begin(), present.end());
+    auto group_key = [&](int v) { return key + "|g" + std::to_string(v); };
+    GroupCfgs best = b->gcfg;
+    bool cached = true;
+    for (auto &pv : present) cached = batch_cache_lookup(group_key(pv.second), &best.g[pv.second]) && cached;
+    if (cached) {
+        b->gcfg = best;
+        b->cfg = best.g[present[0].second];
+        if (chosen) *chosen = b->cfg;
         return build_work(b);
     }
     // save the state: roles, clock and every instance's state_old
@@ -670,6 +741,7 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
     std::memcpy(saved_role, b->role, sizeof(saved_role));
     const int64_t saved_steps = b->steps;
     const std::vector<double> saved_time = b->time;
+    const GroupCfgs saved_gcfg = b->gcfg;
     const tsp_batch_cfg saved_cfg = b->cfg;
     int64_t backup_elems = 0;
     for (int i = 0; i < b->count; ++i) backup_elems += b->L[i].elems;
@@ -700,9 +772,9 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
             for (int gb : {0, 1024})
                 for (int hr = 0; hr < 2; ++hr)
                     for (int nt = 0; nt < 2; ++nt) cands.push_back(tsp_batch_cfg{ty, zc, gb, 1, hr, nt});
-    // mean ms per step of `c` over n graph-replayed steps, from the saved state
-    auto time_cfg = [&](const tsp_batch_cfg &c, int n, double *ms) -> tsp_status {
-        b->cfg = c;
+    // mean ms per step of group v with configuration c over n graph-replayed steps, from the saved state
+    auto time_cfg = [&](int v, const tsp_batch_cfg &c, int n, double *ms) -> tsp_status {
+        b->gcfg.g[v] = c;
         tsp_status r = build_work(b);
         if (r == TSP_OK) r = tsp_batch_graph_rk4_steps(b, 2); // capture and warm up
         if (r != TSP_OK) return r;
@@ -720,30 +792,35 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
         e = copy_old(false);
         return e == hipSuccess ? TSP_OK : hip_fail(e, "batch autotune restore");
     };
-    // every candidate over `steps` steps, then the three fastest over three times as
-    // many, each also with zalt 2 (stages 1 and 3 reverse the march)
-    std::vector<std::pair<double, int>> ranked;
-    for (size_t i = 0; i < cands.size() && st == TSP_OK; ++i) {
-        double ms = 0.0;
-        st = time_cfg(cands[i], steps, &ms);
-        ranked.push_back({ms, (int)i});
-    }
-    double best_ms = 1e300;
-    if (st == TSP_OK) {
+    // per group: every candidate over `steps` steps, then the three fastest over three
+    // times as many, each also with zalt 2 (stages 1 and 3 reverse the march)
+    std::vector<double> best_ms(tsp::kBatchGroups, 1e300);
+    for (size_t gi = 0; gi < present.size() && st == TSP_OK; ++gi) {
+        const int v = present[gi].second;
+        b->only_group = v;
+        std::vector<std::pair<double, int>> ranked;
+        for (size_t i = 0; i < cands.size() && st == TSP_OK; ++i) {
+            double ms = 0.0;
+            st = time_cfg(v, cands[i], steps, &ms);
+            ranked.push_back({ms, (int)i});
+        }
+        if (st != TSP_OK) break;
         std::sort(ranked.begin(), ranked.end());
         for (size_t q = 0; q < std::min<size_t>(3, ranked.size()) && st == TSP_OK; ++q)
             for (int za : {1, 2}) {
                 tsp_batch_cfg c = cands[ranked[q].second];
                 c.zalt = za;
                 double ms = 0.0;
-                st = time_cfg(c, 3 * steps, &ms);
+                st = time_cfg(v, c, 3 * steps, &ms);
                 if (st != TSP_OK) break;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = c;
+                if (ms < best_ms[v]) {
+                    best_ms[v] = ms;
+                    best.g[v] = c;
                 }
             }
+        b->gcfg.g[v] = best.g[v]; // the later groups are timed beside nothing else, but build with it
     }
+    b->only_group = -1;
     // restore (the state is already restored after every candidate)
     std::memcpy(b->role, saved_role, sizeof(saved_role));
     b->steps = saved_steps;
@@ -753,12 +830,18 @@ tsp_status tsp_batch_autotune(tsp_batch *b, int32_t steps, tsp_batch_cfg *chosen
     (void)hipFree(backup);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    b->cfg = st == TSP_OK ? best : saved_cfg;
+    if (st == TSP_OK) {
+        b->gcfg = best;
+        b->cfg = best.g[present[0].second];
+    } else {
+        b->gcfg = saved_gcfg;
+        b->cfg = saved_cfg;
+    }
     tsp_status st2 = build_work(b);
     if (st != TSP_OK) return st;
     if (st2 != TSP_OK) return st2;
-    batch_cache_store(key, best, best_ms);
-    if (chosen) *chosen = best;
+    for (auto &pv : present) batch_cache_store(group_key(pv.second), best.g[pv.second], best_ms[pv.second]);
+    if (chosen) *chosen = b->cfg;
     return TSP_OK;
 }
 
@@ -941,8 +1024,7 @@ struct tsp_batch_adjoint {
     int groles0[4] = {-1, -1, -1, -1}; // roles0 the forward graphs were captured for
     hipGraphExec_t fgraph[2] = {nullptr, nullptr}; // two (step, gather) pairs per forward parity
     hipGraphExec_t bgraph[2] = {nullptr, nullptr}; // two (inject, reverse step) pairs per backward parity
-    tsp_batch_cfg gcfg{-1, -1, -1, -1, -1, -1};
-    uint64_t gwork = ~0ull;
+    uint64_t gwork = ~0ull; // work-list generation the graphs and the adjoint instance table were made for
 };
 
 namespace {
@@ -955,14 +1037,19 @@ void drop_batch_adjoint_graphs(tsp_batch_adjoint *a) {
         }
 }
 
-// graphs hold the batch's work list and configuration
+// Graphs hold the batch's work list, and the adjoint instance table the z
+// chunk of every instance; both follow a rebuilt work list (every change of a
+// group's configuration rebuilds it).
 tsp_status batch_adjoint_check(tsp_batch_adjoint *a) {
     tsp_status st = build_work(a->b);
     if (st != TSP_OK) return st;
-    if (a->gwork != a->b->work_gen || std::memcmp(&a->gcfg, &a->b->cfg, sizeof(tsp_batch_cfg)) != 0) {
+    if (a->gwork != a->b->work_gen) {
         drop_batch_adjoint_graphs(a);
+        for (int i = 0; i < a->b->count; ++i) a->h_ainst[i].zchunk = a->b->h_inst[i].zchunk;
+        TSP_HIP(hipMemcpyAsync(a->d_ainst, a->h_ainst.data(), sizeof(tsp::BatchInstance) * a->b->count,
+                               hipMemcpyHostToDevice, a->b->stream));
+        TSP_HIP(hipStreamSynchronize(a->b->stream));
         a->gwork = a->b->work_gen;
-        a->gcfg = a->b->cfg;
     }
     return TSP_OK;
 }
@@ -994,10 +1081,11 @@ tsp_status enqueue_batch_adjoint_step(tsp_batch_adjoint *a) {
             g.init = OLD;
             g.acc = NEW;
             g.argn = j > 0 ? ARGN : -1;
-            g.zalt = b->cfg.zalt;
+            const tsp_batch_cfg &gc = b->gcfg.g[v];
+            g.zalt = gc.zalt;
             g.ghost_mode = 0;
-            const int grid = b->cfg.grid_blocks > 0 ? std::min(b->cfg.grid_blocks, g.nwork) : g.nwork;
-            TSP_HIP(tsp::launch_batched_phipi_adjoint(v, g, b->cfg.tile_y, grid, b->max_abl_T, b->cfg.halo_ring,
+            const int grid = gc.grid_blocks > 0 ? std::min(gc.grid_blocks, g.nwork) : g.nwork;
+            TSP_HIP(tsp::launch_batched_phipi_adjoint(v, g, gc.tile_y, grid, b->max_abl_T, gc.halo_ring,
                                                       b->stream));
         }
     }

@@ -263,6 +263,23 @@ class Batch:
         c = _lib.TspBatchCfg(**cur)
         _lib.check(_lib.lib().tsp_batch_set_kernel(self._h, C.byref(c)), "tsp_batch_set_kernel")
 
+    def group_kernel(self, stencil_n: int, ko_order: int, system: str = "phipi") -> dict:
+        """Kernel configuration of one (system, stencil width, KO order) group."""
+        cfg = _lib.TspBatchCfg()
+        _lib.check(_lib.lib().tsp_batch_get_group_kernel(self._h, _lib.SYSTEMS[system], int(stencil_n),
+                                                           int(ko_order), C.byref(cfg)),
+                   "tsp_batch_get_group_kernel")
+        return cfg.as_dict()
+
+    def set_group_kernel(self, stencil_n: int, ko_order: int, cfg: dict, system: str = "phipi") -> None:
+        """Set one group's kernel configuration (fields not given keep their current values)."""
+        cur = self.group_kernel(stencil_n, ko_order, system)
+        cur.update(cfg)
+        c = _lib.TspBatchCfg(**cur)
+        _lib.check(_lib.lib().tsp_batch_set_group_kernel(self._h, _lib.SYSTEMS[system], int(stencil_n),
+                                                           int(ko_order), C.byref(c)),
+                   "tsp_batch_set_group_kernel")
+
     def rk4_steps(self, nsteps: int) -> None:
         _lib.check(_lib.lib().tsp_batch_rk4_steps(self._h, int(nsteps)), "tsp_batch_rk4_steps")
 
@@ -286,7 +303,8 @@ class Batch:
         _lib.check(_lib.lib().tsp_batch_euler_steps(self._h, int(nsteps)), "tsp_batch_euler_steps")
 
     def autotune(self, steps: int = 6) -> dict:
-        """The library's batch autotuner (tsp_batch_autotune); returns the chosen configuration."""
+        """The library's batch autotuner (tsp_batch_autotune): every (system, stencil width, KO order)
+        group tuned on its own; returns the largest group's configuration (group_kernel gives the others)."""
         cfg = _lib.TspBatchCfg()
         _lib.check(_lib.lib().tsp_batch_autotune(self._h, int(steps), C.byref(cfg)), "tsp_batch_autotune")
         return cfg.as_dict()
