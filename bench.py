@@ -167,7 +167,7 @@ def ensemble_leg(count: int, steps: int, tune: bool):
     }
 
 
-def tensor_leg(n: int, steps: int):
+def tensor_leg(n: int, steps: int, tune: bool = True):
     """Linearised ADM (h_ij, K_ij) workload: 12 comps at n^3, n=3 stencils with mixed second
     derivatives, KO order 3, 12-cell ABL; graph-replayed steps."""
     import torch
@@ -178,6 +178,7 @@ def tensor_leg(n: int, steps: int):
     case = _tensor_case(n)
     s = Solver(case)
     s.set_state(fourier_state_device(case, "tensor192"))
+    cfg = s.autotune(2) if tune else s.kernel  # block shape and z-slab depth of the block order
     s.graph_rk4_steps(2)
     ms = s.time_rk4(steps, use_graph=True)
     stage_ms, fill_ms = s.profile_rk4(2)
@@ -193,6 +194,8 @@ def tensor_leg(n: int, steps: int):
         "algorithmic_GBs": cu * 128 * 12 / 1e9,
         "roofline_frac": cu * 128 * 12 / 8e12,
         "stage_ms": [x / 2 for x in stage_ms],
+        "kernel": {"block": {4: "8x8x4", 16: "16x8x4"}.get(cfg["tile_y"], "16x8x2"), "slab_z_tiles": cfg["zchunk"],
+                   "chosen_by": "tsp_autotune" if tune else "default"},
         "ghost_fill_ms_per_step": fill_ms / 2,
     }
 
@@ -398,7 +401,7 @@ def main() -> int:
     if not args.no_tensor:
         log("[bench] tensor leg: 192^3 x 12 comps ...")
         t0 = time.perf_counter()
-        ten = tensor_leg(192, 20)
+        ten = tensor_leg(192, 20, tune=not args.no_tune)
         log(f"[bench] tensor: {ten['cell_updates_per_s']:.3e} cu/s, {ten['algorithmic_GBs']:.0f} GB/s algorithmic "
             f"({100 * ten['roofline_frac']:.1f}%), stage ms {[round(x, 2) for x in ten['stage_ms']]} "
             f"({time.perf_counter() - t0:.1f} s)")

@@ -97,8 +97,11 @@ typedef struct tsp_desc {
 typedef struct tsp_kernel_cfg {
     int32_t ghost_mode; /* 0: virtual ghosts (no fill pass); 1: FillPatch pass then plain reads */
     int32_t tile_y;     /* rows per workgroup: 2, 4, 8 or 16 with 64-wide tiles (one wave per row),
-                           or 32 for 32 x 32 tiles (two rows per wave) */
-    int32_t zchunk;     /* z planes marched per workgroup (>= 1) */
+                           or 32 for 32 x 32 tiles (two rows per wave). The tensor system's block
+                           shape: 8 -> 16 x 8 x 2 cells, 4 -> 8 x 8 x 4, 16 -> 16 x 8 x 4 (512
+                           threads), others 16 x 8 x 2 */
+    int32_t zchunk;     /* z planes marched per workgroup (>= 1); the tensor system: z tiles per
+                           slab of its XCD-wise block order */
     int32_t xcd_remap;  /* 1: map consecutive tiles onto one XCD (shared L2) */
     int32_t zalt;       /* 1: odd z chunks march downwards (chunk-boundary planes read together);
                            2: as 1, and RK4 stages 1 and 3 reverse every direction (a stage

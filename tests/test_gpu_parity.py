@@ -127,6 +127,31 @@ def test_every_tile_shape_is_bitwise_identical():
         assert np.array_equal(o, want)
 
 
+@pytest.mark.parametrize("stencil_n, ko_order", [(3, 3), (2, 0), (4, 4)])
+def test_tensor_block_shapes_are_bitwise_identical(stencil_n, ko_order):
+    """The tensor kernel's block shapes (tile_y 8: 16 x 8 x 2, 4: 8 x 8 x 4, 16: 16 x 8 x 4) and
+    z-slab depths of its block order (zchunk) all equal the oracle bitwise, on a box ragged
+    against every shape; tsp_autotune picks among them and leaves the state as it was."""
+    _say(f"[parity] tensor block shapes n={stencil_n} ko={ko_order}")
+    case, state = small_case("tensor", n=(21, 19, 13), stencil_n=stencil_n, ko_order=ko_order,
+                             periodic=(1, 0, 0), abl=3, seed=31)
+    want = None
+    for ty in (8, 4, 16):
+        for zc in (1, 2, 3):
+            got, want = run_both(case, state, 2, kernel={"ghost_mode": 1, "tile_y": ty, "zchunk": zc})
+            assert np.array_equal(got, want), (ty, zc)
+    s = Solver(case)
+    s.set_state(state)
+    s.rk4_steps(1)
+    before = s.get_state()
+    cfg = s.autotune(1)
+    assert cfg["tile_y"] in (4, 8, 16) and cfg["zchunk"] in (1, 2, 4) and s.steps == 1
+    assert np.array_equal(s.get_state(), before)
+    s.rk4_steps(1)
+    assert np.array_equal(s.get_state(), want)
+    s.close()
+
+
 @pytest.mark.parametrize("nsteps", [1, 2, 5, 6])
 def test_graph_replay_equals_eager(nsteps):
     _say(f"[parity] graph replay nsteps={nsteps}")

@@ -68,6 +68,8 @@ namespace {
 // tall enough, and the longest z chunk (of 64, 32, 16, 8) that still gives at
 // least two workgroups per CU, so small boxes fill the device too.
 tsp_kernel_cfg default_cfg(const tsp_desc &d) {
+    // the tensor kernel: 16 x 8 x 2 blocks, block order in slabs of two z tiles
+    if (d.system == TSP_SYS_TENSOR_ADM) return tsp_kernel_cfg{1, 8, 2, 1, 0, 0, 0};
     tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1, 0};
     // mixed stencils read filled edge/corner ghosts; a custom kernel reads ghost memory
     if (d.system == TSP_SYS_TENSOR_ADM || d.system == TSP_SYS_CUSTOM) c.ghost_mode = 1;
@@ -966,10 +968,6 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     const std::string key = tune_key(s);
     tsp_kernel_cfg best = s->cfg;
     if (s->d.system == TSP_SYS_CUSTOM) return fail(TSP_UNSUPPORTED, "a custom model has no library stage kernel to tune");
-    if (s->d.system == TSP_SYS_TENSOR_ADM) { // one launch shape; nothing to tune
-        if (chosen) *chosen = best;
-        return TSP_OK;
-    }
     if (cache_lookup(s, key, &best)) {
         s->cfg = best;
         if (chosen) *chosen = best;
@@ -1001,11 +999,18 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
     // fastest four are timed again over more steps, then the best geometry
     // with the fill-pass mode
     std::vector<tsp_kernel_cfg> cands;
-    for (int ty : {2, 4, 8, 16, 32})
-        for (int zc : zchunks)
-            for (int xr = 0; xr < 2; ++xr)
-                for (int za = 0; za < 3; ++za)
-                    for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr) cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr, 0});
+    const bool tensor = s->d.system == TSP_SYS_TENSOR_ADM;
+    if (tensor) { // block shape (tile_y) and z tiles per slab of the block order (zchunk)
+        for (int ty : {8, 4, 16})
+            for (int zc : {1, 2, 4}) cands.push_back(tsp_kernel_cfg{1, ty, zc, 1, 0, 0, 0});
+    } else {
+        for (int ty : {2, 4, 8, 16, 32})
+            for (int zc : zchunks)
+                for (int xr = 0; xr < 2; ++xr)
+                    for (int za = 0; za < 3; ++za)
+                        for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr)
+                            cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr, 0});
+    }
     // stage pairs (32 x 32 tiles; tile_y, zalt and halo_ring do not apply)
     if (pairs_apply(s, 0, nullptr))
         for (int zc : zchunks)
@@ -1057,7 +1062,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
             best_ms = fine[bi];
         }
     }
-    if (st == TSP_OK) {
+    if (st == TSP_OK && !tensor) { // (the tensor kernel always runs after the fill pass)
         tsp_kernel_cfg fill = best;
         fill.ghost_mode = 1;
         fill.stage_pairs = 0;

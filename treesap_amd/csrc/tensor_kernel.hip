@@ -38,9 +38,10 @@ __device__ __forceinline__ double pick3(int m, double v0, double v1, double v2) 
     return m == 0 ? v0 : m == 1 ? v1 : v2;
 }
 
-template <int N, int R> struct TensorTile {
-    static constexpr int TX = 16, TY = 8, TZ = 2, NT = TX * TY * TZ; // 256 threads: two workgroups per CU
-    static constexpr int ZG = 2; // z tiles per slab of the block order
+// Block shapes (x, y, z cells, one per thread): 16 x 8 x 2 and 8 x 8 x 4
+// (256 threads, two workgroups per CU), 16 x 8 x 4 (512 threads, one).
+template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
+    static constexpr int TX = BX, TY = BY, TZ = BZ, NT = TX * TY * TZ;
     static constexpr int H = N > R ? N : R;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, REG = RX * RY * RZ;
 
@@ -152,9 +153,10 @@ template <int N, int R> struct TensorTile {
         }
         // Each XCD's blocks (a contiguous range of bid after the remap) cover one
         // group of GS consecutive (x, y) tiles (x fastest) over all z, swept in
-        // slabs of ZG z tiles: a slab's GS * ZG blocks, about what the XCD's
-        // CUs hold at once, share their halos in the XCD's L2, and the next
-        // slab rereads the previous one's last planes from there.
+        // slabs of ZG = a.zchunk z tiles: a slab's GS * ZG blocks, about what
+        // the XCD's CUs hold at once, share their halos in the XCD's L2, and the
+        // next slab rereads the previous one's last planes from there.
+        const int ZG = max(a.zchunk, 1);
         const int tiles_y = (a.ny + TY - 1) / TY;
         const int tiles_z = (a.nz + TZ - 1) / TZ;
         const int nxy = tiles_x * tiles_y;
@@ -376,22 +378,28 @@ template <int N, int R> struct TensorTile {
     }
 };
 
-template <int N, int R>
-__global__ __launch_bounds__((TensorTile<N, R>::NT)) void tensor_stage_kernel(const StageArgs a) {
-    __shared__ double lds[TensorTile<N, R>::LDS_DOUBLES];
-    TensorTile<N, R>::run(a, lds);
+template <int N, int R, int BX, int BY, int BZ>
+__global__ __launch_bounds__((TensorTile<N, R, BX, BY, BZ>::NT)) void tensor_stage_kernel(const StageArgs a) {
+    __shared__ double lds[TensorTile<N, R, BX, BY, BZ>::LDS_DOUBLES];
+    TensorTile<N, R, BX, BY, BZ>::run(a, lds);
+}
+
+template <int N, int R, int BX, int BY, int BZ> hipError_t launch_tensor_shape(const StageArgs &a, hipStream_t stream) {
+    using TT = TensorTile<N, R, BX, BY, BZ>;
+    const long long blocks = (long long)((a.nx + TT::TX - 1) / TT::TX) * ((a.ny + TT::TY - 1) / TT::TY) *
+                             ((a.nz + TT::TZ - 1) / TT::TZ);
+    if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tensor_stage_kernel<N, R, BX, BY, BZ>), dim3((unsigned)blocks), dim3(TT::NT), 0, stream, a);
+    return hipGetLastError();
 }
 
 } // namespace
 
+// The block shape by tile_y: 4 -> 8 x 8 x 4, 16 -> 16 x 8 x 4, otherwise 16 x 8 x 2.
 template <int N, int R> hipError_t launch_tensor(const StageArgs &a, int tile_y, hipStream_t stream) {
-    using TT = TensorTile<N, R>;
-    (void)tile_y;
-    const long long blocks = (long long)((a.nx + TT::TX - 1) / TT::TX) * ((a.ny + TT::TY - 1) / TT::TY) *
-                             ((a.nz + TT::TZ - 1) / TT::TZ);
-    if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tensor_stage_kernel<N, R>), dim3((unsigned)blocks), dim3(TT::NT), 0, stream, a);
-    return hipGetLastError();
+    if (tile_y == 4) return launch_tensor_shape<N, R, 8, 8, 4>(a, stream);
+    if (tile_y == 16) return launch_tensor_shape<N, R, 16, 8, 4>(a, stream);
+    return launch_tensor_shape<N, R, 16, 8, 2>(a, stream);
 }
 
 LaunchFn lookup_tensor(int n, int r) {
