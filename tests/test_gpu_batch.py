@@ -227,6 +227,45 @@ def test_group_kernels_run_their_own_configurations():
     b.close()
 
 
+def test_batch_tune_cache_holds_one_line_per_group(monkeypatch, tmp_path):
+    """tsp_batch_autotune stores one cache line per (system, stencil width, KO order) group; a
+    second batch of the same shape takes every group's configuration from the cache (an edited
+    line wins), and its instances still equal the oracle."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("TSP_TUNE_CACHE", str(cache))
+    _say("[batch] tune cache per group")
+    cases = mixed_cases()[:5]  # groups (4,4), (2,3), (3,3), (1,1), (2,2)
+    b = Batch([c for c, _ in cases])
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.autotune(2)
+    tuned = {(c.stencil_n, c.ko_order): b.group_kernel(c.stencil_n, c.ko_order) for c, _ in cases}
+    b.close()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == len(tuned) and all(len(ln.split()) == 8 for ln in lines)
+    # the (2, 2) group's line, edited to another configuration
+    groups = [ln.split()[0] for ln in lines]
+    key22 = [k for k in groups if k.endswith("|g3")]  # group 3: (phi, Pi) with (stencil_n, ko_order) = (2, 2)
+    assert len(key22) == 1
+    with cache.open("a") as f:
+        f.write(f"{key22[0]} 8 3 17 2 0 0 0.1\n")
+    b = Batch([c for c, _ in cases])
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.autotune(2)
+    for (sn, ko), cfg in tuned.items():
+        want = cfg if (sn, ko) != (2, 2) else dict(tile_y=8, zchunk=3, grid_blocks=17, zalt=2, halo_ring=0,
+                                                    narrow_tiles=0)
+        assert b.group_kernel(sn, ko) == want, (sn, ko)
+    b.graph_rk4_steps(2)
+    for i, (case, st) in enumerate(cases):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(2)
+        assert np.array_equal(b.get_state(i), o.get_state()), i
+    b.close()
+
+
 def test_mixed_system_batch_matches_oracle():
     """(phi, Pi), (phi, u) and NonLinear instances in one batch: one launch per stage and
     (system, stencil width, KO order) group; every instance bitwise equal to its single-box
