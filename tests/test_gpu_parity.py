@@ -313,6 +313,29 @@ def _random_case(rng):
     return system, n, sn, ko, per, abl, cfg
 
 
+# (stencil width, KO order) pairs the tensor kernel supports
+_TENSOR_PAIRS = [(1, 0), (1, 1), (2, 0), (2, 2), (3, 0), (3, 3), (4, 0), (4, 4)]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_tensor_cases_match_oracle(seed):
+    """Seeded random tensor (h, K) cases: box shapes ragged against every block shape, stencil/KO
+    pairs, boundary mixes, absorbing layers, block shape and z-slab depth of the block order, RK4 or
+    Euler; bitwise against the oracle."""
+    rng = np.random.default_rng(3000 + seed)
+    sn, ko = _TENSOR_PAIRS[int(rng.integers(len(_TENSOR_PAIRS)))]
+    H = max(sn, ko)
+    n = (int(rng.integers(max(H, 3), 41)), int(rng.integers(max(H, 3), 29)), int(rng.integers(max(H, 3), 23)))
+    per = tuple(int(v) for v in rng.integers(0, 2, 3))
+    abl = int(rng.integers(1, min(n) // 2)) if min(n) >= 6 and rng.random() < 0.5 else 0
+    cfg = {"ghost_mode": 1, "tile_y": int(rng.choice([4, 8, 16])), "zchunk": int(rng.integers(1, 6))}
+    scheme = "euler" if rng.random() < 0.25 else "rk4"
+    _say(f"[parity] random tensor seed={seed} n={n} stencil={sn} ko={ko} periodic={per} abl={abl} cfg={cfg} {scheme}")
+    case, state = small_case("tensor", n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=90 + seed)
+    got, want = run_both(case, state, 2, scheme=scheme, kernel=cfg)
+    assert np.array_equal(got, want), f"max abs diff {np.max(np.abs(got - want))}"
+
+
 @pytest.mark.parametrize("seed", range(24))
 def test_random_shapes_and_configs_match_oracle(seed):
     """Seeded random systems, box shapes (ragged against every tile shape, z chunks down to one plane,
