@@ -160,6 +160,42 @@ def test_random_batches_match_oracle(seed):
         assert np.array_equal(b.get_state(i), o.get_state()), (i, case.n)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_group_configurations_match_oracle(seed):
+    """Seeded random batches in which every (stencil width, KO order) group runs its own random
+    kernel configuration; every instance bitwise against the oracle after one eager and two
+    graph-replayed RK4 steps."""
+    rng = np.random.default_rng(2500 + seed)
+    cases = []
+    for k in range(int(rng.integers(5, 10))):
+        sn, ko = _BATCH_PAIRS[int(rng.integers(len(_BATCH_PAIRS)))]
+        H = max(sn, ko)
+        n = (int(rng.integers(max(H, 5), 110)), int(rng.integers(max(H, 3), 40)), int(rng.integers(max(H, 3), 24)))
+        per = tuple(int(v) for v in rng.integers(0, 2, 3))
+        abl = int(rng.integers(1, min(n) // 2)) if min(n) >= 6 and rng.random() < 0.4 else 0
+        case, state = small_case("phipi", n=n, stencil_n=sn, ko_order=ko, periodic=per, abl=abl, seed=500 + 10 * seed + k)
+        cases.append((case.replace(dt=case.dt * (1.0 + 0.05 * k)), state))
+    b = Batch([c for c, _ in cases])
+    groups = sorted({(c.stencil_n, c.ko_order) for c, _ in cases})
+    cfgs = {}
+    for sn, ko in groups:
+        cfgs[(sn, ko)] = {"tile_y": int(rng.choice([8, 16])), "zchunk": int(rng.choice([0, 1, 5, 64])),
+                          "grid_blocks": int(rng.choice([0, 7, 300])), "zalt": int(rng.integers(3)),
+                          "halo_ring": int(rng.integers(2)), "narrow_tiles": int(rng.integers(2))}
+        b.set_group_kernel(sn, ko, cfgs[(sn, ko)])
+    _say(f"[batch] random group configurations seed={seed} {[c.n for c, _ in cases]} {cfgs}")
+    for i, (_, st) in enumerate(cases):
+        b.set_state(i, st)
+    b.rk4_steps(1)
+    b.graph_rk4_steps(2)
+    for i, (case, st) in enumerate(cases):
+        o = orc.OracleSolver(case)
+        o.set_state(st)
+        o.rk4_steps(3)
+        assert np.array_equal(b.get_state(i), o.get_state()), (i, case.n, case.stencil_n, case.ko_order)
+    b.close()
+
+
 def test_batch_autotune_restores_state_and_keeps_results():
     """tsp_batch_autotune times its candidates on the batch itself: afterwards every instance's
     state, the step count and the clock are what they were, and stepping with the chosen
