@@ -200,6 +200,27 @@ def tensor_leg(n: int, steps: int, tune: bool = True):
     }
 
 
+def generic_leg(n: int, steps: int):
+    """The stage kernel compiled from a user functor (include/treesap/device/generic.hpp), timed by the
+    hipcc-built program tests/cpp/bin/generic_timing (a child process) on the wave256 arithmetic written
+    as a functor the library does not carry: (phi, Pi) with n=4 stencils, KO order 4 sigma=0.2 and a
+    16-cell quadratic ABL."""
+    import subprocess
+
+    prog = ROOT / "tests" / "cpp" / "bin" / "generic_timing"
+    if not prog.exists():
+        raise RuntimeError(f"{prog} is missing; build it with tests/cpp/build.py")
+    res = subprocess.run([str(prog), str(n), str(steps)], capture_output=True, text=True, timeout=300)
+    if res.returncode != 0:
+        raise RuntimeError(f"generic_timing failed: {res.stderr.strip()}")
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    if not out["finite"]:
+        raise RuntimeError("generic leg: non-finite state after the timed steps")
+    out["workload"] = (f"wave{n} arithmetic as a user functor (KODecorator<ScalarWave<4,4>,4> under ABLPDEModel), "
+                       "stage kernel compiled from the functor, graph-replayed RK4 steps")
+    return out
+
+
 def _tensor_case(n: int):
     from treesap_amd.workloads import Case
 
@@ -297,6 +318,7 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-vs-plain ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward leg")
     ap.add_argument("--no-tensor", action="store_true", help="skip the 12-component tensor leg")
+    ap.add_argument("--no-generic", action="store_true", help="skip the user-functor (generic kernel) leg")
     ap.add_argument("--ensemble-count", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -406,6 +428,15 @@ def main() -> int:
             f"({100 * ten['roofline_frac']:.1f}%), stage ms {[round(x, 2) for x in ten['stage_ms']]} "
             f"({time.perf_counter() - t0:.1f} s)")
 
+    gen = None
+    if not args.no_generic:
+        log("[bench] generic leg: the wave arithmetic as a user functor ...")
+        t0 = time.perf_counter()
+        gen = generic_leg(args.n, 20)
+        gen["vs_registered_kernels"] = gen["cell_updates_per_s"] / value
+        log(f"[bench] generic: {gen['cell_updates_per_s']:.3e} cu/s ({gen['vs_registered_kernels']:.2f} of the "
+            f"registered kernels) ({time.perf_counter() - t0:.1f} s)")
+
     adj = None
     if not args.no_adjoint:
         log("[bench] adjoint leg: 128^3 x 16 sources x 500 steps ...")
@@ -460,6 +491,7 @@ def main() -> int:
         "batched": ens,
         "adjoint": adj,
         "tensor": ten,
+        "generic": gen,
     }
     print(json.dumps(line), flush=True)
     return 0
