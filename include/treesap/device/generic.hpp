@@ -76,10 +76,12 @@ __global__ __launch_bounds__(256) void generic_stage_kernel(const RHS rhs, const
     });
 }
 
-// tsp_stage_fn: launch the stage kernel of `user` (a GenericStage<RHS>) on the solver's stream
+// tsp_stage_fn: launch the stage kernel of `user` (a GenericStage<RHS>) on the solver's stream;
+// returns 0, or 1 for an unknown stage, 2 for a failed launch, 3 for a box whose y or z
+// extent exceeds the launch grid (more than 4 * 65535 cells along y or 65535 along z)
 template <class RHS> int stage_callback(void *user, const tsp_stage_view *v, void *stream) {
     const auto *g = static_cast<const GenericStage<RHS> *>(user);
-    if ((v->n[1] + 3) / 4 > 65535 || v->n[2] > 65535) return 1; // beyond the grid's y and z extents
+    if ((v->n[1] + 3) / 4 > 65535 || v->n[2] > 65535) return 3; // beyond the grid's y and z extents
     const dim3 blocks((v->n[0] + 63) / 64, (v->n[1] + 3) / 4, v->n[2]), threads(64, 4);
     const hipStream_t st = (hipStream_t)stream;
     switch (v->stage) {
