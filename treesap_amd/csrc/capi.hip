@@ -47,6 +47,9 @@ struct tsp_solver {
     // that registers of 4 GiB or more use, on any size (for testing it)
     bool ptr_access = false;
     double *sigma_tab = nullptr;
+    // tensor system with KO: the K components' KO sums of the current stage
+    // (six components in the register layout, written and read by the stage launch)
+    double *kko = nullptr;
     tsp::LaunchFn launch = nullptr;
     tsp::LaunchFn launch_stage[4] = {nullptr, nullptr, nullptr, nullptr}; // stage-specific, where built
     tsp::PairFn launch_pair = nullptr;                                      // stage pairs, where built
@@ -155,6 +158,7 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     a.init = reg_origin(s, init_reg);
     a.acc = reg_origin(s, acc_reg);
     a.argn = argn_reg >= 0 ? reg_origin(s, argn_reg) : nullptr;
+    a.kko = s->kko ? s->kko + kXOff + (int64_t)s->G * s->sj + (int64_t)s->G * s->sk : nullptr;
     a.sigma_tab = s->sigma_tab;
     a.nx = s->d.n[0];
     a.ny = s->d.n[1];
@@ -412,6 +416,14 @@ tsp_status tsp_create(const tsp_desc *d, void *stream, tsp_solver **out) {
         e = hipMemsetAsync(s->base[r], 0, sizeof(double) * s->reg_elems, s->stream);
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemsetAsync"));
     }
+    if (d->system == TSP_SYS_TENSOR_ADM && d->ko_order > 0) {
+        const size_t bytes = sizeof(double) * 6 * (size_t)s->sc;
+        hipError_t e = hipMalloc((void **)&s->kko, bytes);
+        if (e == hipErrorOutOfMemory) return cleanup(fail(TSP_OOM, "hipMalloc of the KO buffer failed (out of memory)"));
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc"));
+        e = hipMemsetAsync(s->kko, 0, bytes, s->stream);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemsetAsync"));
+    }
     // Ghost cells outside a non-periodic face only ever hold the boundary
     // values (stages write valid cells only), so they are written once here
     // for every register; the virtual-ghost stage kernels read them as is.
@@ -449,6 +461,7 @@ tsp_status tsp_destroy(tsp_solver *s) {
     for (int r = 0; r < 4; ++r)
         if (s->base[r]) (void)hipFree(s->base[r]);
     if (s->sigma_tab) (void)hipFree(s->sigma_tab);
+    if (s->kko) (void)hipFree(s->kko);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return TSP_OK;

@@ -107,6 +107,7 @@ struct StageArgs {
     const double *init;      // state_old (centre reads)
     double *acc;             // state_new
     double *argn;            // arg_next
+    double *kko;             // tensor system with KO: KO sums of the K components (register layout, 6 components)
     const double *sigma_tab; // ABL sigma by squared distance to the bulk box
     int32_t nx, ny, nz;
     int32_t G;               // ghost width of the allocation

@@ -17,10 +17,13 @@
 // reference's order: each accumulator receives at most one term per component.
 //
 // Tiling. A workgroup owns a 16 x 8 x 2 block of cells (one per thread; at
-// ~250 VGPRs two such workgroups share a CU, so one computes while the other
-// waits at a barrier) and visits the components one at a time. For each it stages the block plus an
-// H-wide halo of the (ghost-filled) source in LDS. The three unmixed second
-// derivatives and the KO terms are read from there. For the six mixed ones,
+// ~230 VGPRs two such workgroups share a CU, so one computes while the other
+// waits at a barrier) and visits the h components one at a time. For each it
+// stages the block plus an H-wide halo of the (ghost-filled) source in LDS.
+// The three unmixed second derivatives and the KO term are read from there.
+// The K components' KO terms are star stencils that need no block region:
+// a separate z-marching kernel evaluates them just before (TensorKO below),
+// and the block kernel reads them back at its K updates. For the six mixed ones,
 // the inner first derivative g_i = D_i h is first evaluated at every point an
 // outer stencil reads (a "slab": the block extended by n along the outer
 // axis) and kept in LDS; the outer derivative then reads g_i. g_i at a shifted
@@ -57,10 +60,7 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
     static constexpr int P1 = ssize<1> + 2 * ssize<0>;
     static constexpr int P2 = 2 * ssize<2> + ssize<1>;
     static constexpr int GBUF = P1 + P2;
-    // with KO (R > 0): a second region buffer for the K component whose KO
-    // term is evaluated in the same pass, and six KO terms per thread
-    static constexpr int KBUF = R > 0 ? REG + 6 * NT : 0;
-    static constexpr int LDS_DOUBLES = REG + GBUF + KBUF;
+    static constexpr int LDS_DOUBLES = REG + GBUF;
 
     static __device__ __forceinline__ int ridx(int rx, int ry, int rz) { return (rz * RY + ry) * RX + rx; }
 
@@ -138,8 +138,6 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
     static __device__ void run(const StageArgs &a, double *lds) {
         double *u = lds;
         double *g = lds + REG;
-        double *v = g + GBUF;   // K component region (R > 0)
-        double *kob = v + REG;  // KO terms of the K components, [c - 6][tid] (R > 0)
         const int tid = threadIdx.x;
         const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
         const int tiles_x = (a.nx + TX - 1) / TX;
@@ -197,12 +195,11 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
         // start of that component's pass, so its latency overlaps the previous
         // component's arithmetic.
         // Not every region point is read: no stencil reaches a corner of the
-        // region (outside the block along all three axes), and the K
-        // components' KO stencils read only its star (outside along at most
-        // one axis). The point sets are bit masks over the thread's points.
+        // region (outside the block along all three axes). The point set is a
+        // bit mask over the thread's points.
         constexpr int RPT = (REG + NT - 1) / NT;
         int roff[RPT];
-        unsigned hmask = 0, kmask = 0;
+        unsigned hmask = 0;
 #pragma unroll
         for (int m = 0; m < RPT; ++m) {
             const int p = min(tid + m * NT, REG - 1);
@@ -212,24 +209,19 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
             const int z = min(z0 + rz - H, a.nz + a.G - 1);
             roff[m] = x + y * a.sj + z * a.sk;
             const int outside = (rx < H || rx >= H + TX) + (ry < H || ry >= H + TY) + (rz < H || rz >= H + TZ);
-            if (tid + m * NT < REG) {
-                if (outside < 3) hmask |= 1u << m;
-                if (outside < 2) kmask |= 1u << m;
-            }
+            if (tid + m * NT < REG && outside < 3) hmask |= 1u << m;
         }
-        double pre[RPT], prek[RPT];
+        double pre[RPT];
         auto load_region = [&](int c, double (&dst)[RPT]) {
             const double *src = a.src + (long long)c * a.sc;
-            const unsigned mask = c < 6 ? hmask : kmask;
 #pragma unroll
             for (int m = 0; m < RPT; ++m)
-                if (mask >> m & 1u) dst[m] = src[roff[m]];
+                if (hmask >> m & 1u) dst[m] = src[roff[m]];
         };
-        auto stage_region = [&](int c, const double (&from)[RPT], double *buf) {
-            const unsigned mask = c < 6 ? hmask : kmask;
+        auto stage_region = [&](const double (&from)[RPT], double *buf) {
 #pragma unroll
             for (int m = 0; m < RPT; ++m)
-                if (mask >> m & 1u) buf[tid + m * NT] = from[m];
+                if (hmask >> m & 1u) buf[tid + m * NT] = from[m];
         };
         // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154)
         auto update = [&](int c, double kv, double cenc, double initv, double accv) {
@@ -264,18 +256,12 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
             });
         });
 
-        // h components in increasing order (the reference's contraction
-        // order). With KO, pass c also stages K component c's region and
-        // evaluates its KO term (kept in LDS until the K updates), so the K
-        // components need no passes of their own: a pass with little
-        // arithmetic would wait out the latency of every region load.
+        // h components in increasing order (the reference's contraction order)
         load_region(0, pre);
-        if constexpr (R > 0) load_region(6, prek);
 #pragma unroll 1
         for (int c = 0; c < 6; ++c) {
-            __syncthreads(); // the previous component's reads of u, v and g are done
-            stage_region(c, pre, u);
-            if constexpr (R > 0) stage_region(6 + c, prek, v);
+            __syncthreads(); // the previous component's reads of u and g are done
+            stage_region(pre, u);
             // centre values first, then the prefetch: the memory counter
             // completes in order, so the update's wait for the centre values
             // then leaves the next region's loads in flight
@@ -285,10 +271,7 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
                 if (read_init) initv = a.init[off + (long long)c * a.sc];
                 if (!is_first) accv = a.acc[off + (long long)c * a.sc];
             }
-            if (c < 5) {
-                load_region(c + 1, pre);
-                if constexpr (R > 0) load_region(7 + c, prek);
-            }
+            if (c < 5) load_region(c + 1, pre);
             __syncthreads();
 
             double dv[3][3];
@@ -297,7 +280,6 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
             dv[2][2] = d2<2>(u, own, a);
             const double ko_term = ko_sum(u);
             const double cenc = u[own];
-            if constexpr (R > 0) kob[c * NT + tid] = ko_sum(v); // read back by this thread only
 
             const int row = c >= 3 ? 2 : c >= 1 ? 1 : 0;
             const int col = c - (row * (row + 1)) / 2;
@@ -348,8 +330,9 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
             if (active) update(c, decorate(c, -2.0 * kc, ko_term, cenc), cenc, initv, accv);
         }
 
-        // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap); their centre,
-        // init and acc values are loaded together (unrolled loop)
+        // K components: symmetrise<a,b>(f) - 0.5 * (tdd + lap), plus the KO
+        // sums TensorKO left in a.kko; their centre, init, acc and KO values
+        // are loaded together (unrolled loop)
 #pragma unroll
         for (int c = 6; c < 12; ++c) {
             double ko_term = 0.0, cenc = 0.0, initv = 0.0, accv = 0.0;
@@ -357,8 +340,8 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
                 cenc = a.src[off + (long long)c * a.sc];
                 if (read_init) initv = a.init[off + (long long)c * a.sc];
                 if (!is_first) accv = a.acc[off + (long long)c * a.sc];
+                if constexpr (R > 0) ko_term = a.kko[off + (long long)(c - 6) * a.sc];
             }
-            if constexpr (R > 0) ko_term = kob[(c - 6) * NT + tid];
             const int row = c >= 9 ? 2 : c >= 7 ? 1 : 0;
             const int col = (c - 6) - (row * (row + 1)) / 2;
             double fab = 0.0, fba = 0.0, tv = 0.0, lv = 0.0;
@@ -378,8 +361,158 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
     }
 };
 
+// Kreiss-Oliger sums ((0 + KO_x) + KO_y) + KO_z of the six K components
+// (KODecorator.hpp:58-70, stencil.hpp:77-104), written to a.kko (register
+// layout) for the block kernel's K updates. These star stencils need no block
+// region, so they stream: a workgroup of 64 x KY threads owns a 64 x KY
+// column tile and marches a chunk of zc planes, KG components at a time. Each
+// thread keeps its column's 2R + 1 planes of each component in a register
+// queue (the z stencil); a plane's x and y neighbours pass through LDS, in two
+// plane buffers used alternately, so one barrier per plane suffices. The next
+// plane's halo values and queue heads are loaded while a plane computes
+// (loading them two planes ahead, into two alternating register sets,
+// measured 15 % slower per stage).
+template <int R, int KY, int KG> struct TensorKO {
+    static constexpr int LX = 64 + 2 * R, LY = KY + 2 * R, PL = LX * LY, NT = 64 * KY;
+    static constexpr int LDS_DOUBLES = 2 * KG * PL;
+    static_assert(R >= 1 && KY >= R, "the y halo rows are loaded by the first and last R rows of threads");
+    static_assert(6 % KG == 0, "components are taken in equal groups");
+
+    // the KO term along AX from three of the stencil's values: the centre c,
+    // and u(+s), u(-s) by a functor of s (same operation order as TensorTile::ko)
+    template <int AX, class F> static __device__ __forceinline__ double ko(double c, F &&at, const StageArgs &a) {
+        constexpr double c0 = static_cast<double>(cx::ko_coef(R, 0));
+        constexpr double norm = cx::ko_norm(R);
+        double result = c0 * c;
+        static_for<1, R + 1>([&]<int S>() {
+            constexpr double cs = static_cast<double>(cx::ko_coef(R, S));
+            result += cs * at(S);
+        });
+        return result * norm * a.gf1[AX];
+    }
+
+    static __device__ void run(const StageArgs &a, int zc, double *lds) {
+        const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+        const int tiles_x = (a.nx + 63) / 64, tiles_y = (a.ny + KY - 1) / KY;
+        // blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous
+        // run of tiles (x fastest, then y, then z chunk) so that neighbouring
+        // tiles read their common y halo rows from one L2
+        const int nb = gridDim.x;
+        int bid = blockIdx.x;
+        {
+            const int per = nb / 8, rem = nb % 8, xcd = bid % 8;
+            bid = xcd * per + (xcd < rem ? xcd : rem) + bid / 8;
+        }
+        const int txy = bid % (tiles_x * tiles_y), ch = bid / (tiles_x * tiles_y);
+        const int i = (txy % tiles_x) * 64 + tx, j = (txy / tiles_x) * KY + ty;
+        const int z0 = ch * zc, zend = min(z0 + zc, a.nz);
+        const bool active = i < a.nx && j < a.ny;
+        // Load positions are clamped into the ghost-filled allocation
+        // ([-G, n + G) per axis): the lanes of a partial tile past n + G - 1
+        // feed no active lane's stencil.
+        const int xl = a.nx + a.G - 1, yl = a.ny + a.G - 1;
+        const int ic = min(i, xl), jc = min(j, yl);
+        const int cxy = ic + jc * a.sj; // the thread's column
+        const bool hxlo = tx < R, hxhi = tx >= 64 - R, hylo = ty < R, hyhi = ty >= KY - R;
+        const int hx = (hxlo ? i - R : min(i + R, xl)) + jc * a.sj; // x halo point (lanes within R of an x edge)
+        const int hyl = ic + (j - R) * a.sj;                           // y halo points (rows within R of a y edge)
+        const int hyh = ic + min(j + R, yl) * a.sj;
+        // LDS slots of a plane buffer [LY][LX]: the own cell, then the halo points
+        const int own = (ty + R) * LX + tx + R;
+        const int sx = (ty + R) * LX + (hxlo ? tx : tx + 2 * R);
+        const int syl = ty * LX + tx + R, syh = (ty + 2 * R) * LX + tx + R;
+        const long long sk = a.sk;
+
+#pragma unroll 1
+        for (int g0 = 0; g0 < 6; g0 += KG) {
+            if (g0 > 0) __syncthreads(); // the previous group's reads of the plane buffers are done
+            const double *src[KG];
+#pragma unroll
+            for (int g = 0; g < KG; ++g) src[g] = a.src + (long long)(6 + g0 + g) * a.sc;
+            double q[KG][2 * R + 1], vx[KG], vyl[KG], vyh[KG];
+#pragma unroll
+            for (int g = 0; g < KG; ++g) {
+#pragma unroll
+                for (int m = 0; m <= 2 * R; ++m) q[g][m] = src[g][cxy + (z0 - R + m) * sk];
+                if (hxlo || hxhi) vx[g] = src[g][hx + z0 * sk];
+                if (hylo) vyl[g] = src[g][hyl + z0 * sk];
+                if (hyhi) vyh[g] = src[g][hyh + z0 * sk];
+            }
+#pragma unroll 1
+            for (int z = z0; z < zend; ++z) {
+                double *L = lds + ((z - z0) & 1) * (KG * PL);
+#pragma unroll
+                for (int g = 0; g < KG; ++g) {
+                    double *Lg = L + g * PL;
+                    Lg[own] = q[g][R];
+                    if (hxlo || hxhi) Lg[sx] = vx[g];
+                    if (hylo) Lg[syl] = vyl[g];
+                    if (hyhi) Lg[syh] = vyh[g];
+                }
+                __syncthreads();
+                const bool more = z + 1 < zend;
+                if (more) {
+#pragma unroll
+                    for (int g = 0; g < KG; ++g) {
+                        if (hxlo || hxhi) vx[g] = src[g][hx + (z + 1) * sk];
+                        if (hylo) vyl[g] = src[g][hyl + (z + 1) * sk];
+                        if (hyhi) vyh[g] = src[g][hyh + (z + 1) * sk];
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < KG; ++g) {
+                    const double *Lg = L + g * PL;
+                    const double c = q[g][R];
+                    const double kx = ko<0>(c, [&](int s) { return Lg[own + s] + Lg[own - s]; }, a);
+                    const double ky = ko<1>(c, [&](int s) { return Lg[own + s * LX] + Lg[own - s * LX]; }, a);
+                    const double kz = ko<2>(c, [&](int s) { return q[g][R + s] + q[g][R - s]; }, a);
+                    double sum = 0.0;
+                    sum += kx;
+                    sum += ky;
+                    sum += kz;
+                    if (active) a.kko[i + (long long)j * a.sj + z * sk + (long long)(g0 + g) * a.sc] = sum;
+                }
+#pragma unroll
+                for (int g = 0; g < KG; ++g) {
+#pragma unroll
+                    for (int m = 0; m < 2 * R; ++m) q[g][m] = q[g][m + 1];
+                    if (more) q[g][2 * R] = src[g][cxy + (z + 1 + R) * sk];
+                }
+            }
+        }
+    }
+};
+
+// z tiles of the KO kernel's column chunks, its tile height and component group
+constexpr int kKoChunk = 32, kKoTileY = 8, kKoGroup = 2;
+
+template <int R, int KY, int KG> __global__ __launch_bounds__(64 * KY) void tensor_ko_kernel(const StageArgs a, int zc) {
+    __shared__ double lds[TensorKO<R, KY, KG>::LDS_DOUBLES];
+    TensorKO<R, KY, KG>::run(a, zc, lds);
+}
+
+template <int R> hipError_t launch_tensor_ko(const StageArgs &a, hipStream_t stream) {
+    using KO = TensorKO<R, kKoTileY, kKoGroup>;
+    if (!a.kko || a.G < R) return hipErrorInvalidValue;
+    const int zc = min(kKoChunk, a.nz);
+    const long long blocks =
+        (long long)((a.nx + 63) / 64) * ((a.ny + kKoTileY - 1) / kKoTileY) * ((a.nz + zc - 1) / zc);
+    if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tensor_ko_kernel<R, kKoTileY, kKoGroup>), dim3((unsigned)blocks), dim3(KO::NT), 0, stream, a,
+                       zc);
+    return hipGetLastError();
+}
+
+// 256-thread blocks with stencils up to n = 3 are held to 168 VGPRs, three
+// waves per SIMD (three workgroups per CU by their LDS): the compiler spills
+// almost nothing there, and the extra wave per SIMD hides more of the barrier
+// and load waits than the tighter allocation costs (measured 1.17e9 against
+// 9.8e8 cell-updates/s on the 192^3 workload at the compiler's own 202
+// VGPRs). At n = 4 the bound would spill 40 VGPRs.
 template <int N, int R, int BX, int BY, int BZ>
-__global__ __launch_bounds__((TensorTile<N, R, BX, BY, BZ>::NT)) void tensor_stage_kernel(const StageArgs a) {
+__global__ __launch_bounds__((TensorTile<N, R, BX, BY, BZ>::NT))
+__attribute__((amdgpu_waves_per_eu(TensorTile<N, R, BX, BY, BZ>::NT == 256 && N <= 3 ? 3 : 1))) void tensor_stage_kernel(
+    const StageArgs a) {
     __shared__ double lds[TensorTile<N, R, BX, BY, BZ>::LDS_DOUBLES];
     TensorTile<N, R, BX, BY, BZ>::run(a, lds);
 }
@@ -389,6 +522,10 @@ template <int N, int R, int BX, int BY, int BZ> hipError_t launch_tensor_shape(c
     const long long blocks = (long long)((a.nx + TT::TX - 1) / TT::TX) * ((a.ny + TT::TY - 1) / TT::TY) *
                              ((a.nz + TT::TZ - 1) / TT::TZ);
     if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    if constexpr (R > 0) {
+        const hipError_t e = launch_tensor_ko<R>(a, stream);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((tensor_stage_kernel<N, R, BX, BY, BZ>), dim3((unsigned)blocks), dim3(TT::NT), 0, stream, a);
     return hipGetLastError();
 }
