@@ -16,9 +16,9 @@
 // six h components in increasing order accumulates every sum in exactly the
 // reference's order: each accumulator receives at most one term per component.
 //
-// Tiling. A workgroup owns a 16 x 8 x 2 block of cells (one per thread; at
-// ~230 VGPRs two such workgroups share a CU, so one computes while the other
-// waits at a barrier) and visits the h components one at a time. For each it
+// Tiling. A workgroup owns a 16 x 8 x 2 block of cells (one per thread; three
+// such workgroups share a CU, so some compute while others wait at a barrier)
+// and visits the h components one at a time. For each it
 // stages the block plus an H-wide halo of the (ghost-filled) source in LDS.
 // The three unmixed second derivatives and the KO term are read from there.
 // The K components' KO terms are star stencils that need no block region:
