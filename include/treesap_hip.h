@@ -129,7 +129,9 @@ tsp_status tsp_desc_default(tsp_desc *d, int32_t system, int32_t n);
 /* Replaces SingleLevelRK4Integrator(model, block_size)
  * (integrators/SingleLevelRK4Integrator.hpp:65-67, SingleLevelIntegrator.hpp:47-49):
  * validates the model, allocates the four registers (state_old, state_new,
- * arg, arg_next) in device memory. `stream` is a hipStream_t or NULL (the
+ * arg, arg_next) in device memory (the tensor system with KO dissipation also
+ * a six-component buffer for its K components' KO sums, written and read
+ * within each stage). `stream` is a hipStream_t or NULL (the
  * library then creates its own default-priority stream). The descriptor is
  * validated before any HIP call; with an absorbing layer the device's
  * per-workgroup LDS is then queried, since the stage kernels keep the layer's
