@@ -13,6 +13,10 @@
 // (phi, u) and NonLinear systems run split levels; the tensor system and
 // functors on the generic kernel run the level as one box, which gives the
 // same results.
+// The model's boundary functor: NoOpBC and DirichletBC are the library's own;
+// any other BCConcept functor runs on a kernel compiled from it
+// (device/generic_bc.hpp) after each fill's periodic step, on one box, with
+// steps taken eagerly (each fill passes its own time to the functor).
 // initialise() evaluates the initial condition at cell centres
 // x = prob_lo + (i + 0.5) dx (SingleLevelIntegrator.hpp:75-93): a FunctionInit
 // on the device (its expressions compiled to postfix programs), any other
@@ -28,6 +32,7 @@
 #pragma once
 
 #include "Integrator.hpp"
+#include "device/generic_bc.hpp"
 #include "device/generic_runner.hpp"
 #include "device/model_map.hpp"
 #include "device/solver.hpp"
@@ -57,10 +62,14 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
     using RHSType = typename PDEModelType::rhs_t;
     // whether the library's tiled kernels run this model (else the generic kernel)
     static constexpr bool library_kernels = device::runs_library_kernels<PDEModelType>;
+    // whether the boundary functor runs on a kernel compiled from it
+    static constexpr bool custom_boundary = device::custom_boundary<PDEModelType>;
 
     SingleLevelIntegrator(PDEModelType model_, const amrex::IntVect &block_size)
         : Base(std::move(model_)), solver(make_solver(model, block_size)) {
         if constexpr (!library_kernels) generic.emplace(model.get_rhs(), model.get_geometry().data());
+        if constexpr (custom_boundary)
+            boundary.emplace(model.get_bc(), model.get_geometry().data(), model.get_bc_recs());
         ScopedTimer::set_timing_stream(solver.stream());
     }
     ~SingleLevelIntegrator() override {
@@ -143,7 +152,10 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
             throw std::invalid_argument("treesap: set_state needs ncomp * cells values");
         solver.set_state(u.data());
     }
-    tsp_kernel_cfg autotune(int steps_per_candidate = 3) { return solver.autotune(steps_per_candidate); }
+    tsp_kernel_cfg autotune(int steps_per_candidate = 3) {
+        attach_boundary();
+        return solver.autotune(steps_per_candidate);
+    }
     device::Solver &device_solver() { return solver; }
 
   protected:
@@ -171,8 +183,14 @@ class SingleLevelIntegrator : public Integrator<PDEModelType> {
         return device::Solver(device::make_desc(m), block);
     }
 
+    // before stepping: a custom boundary functor's callback, with this integrator's copy
+    void attach_boundary() {
+        if constexpr (custom_boundary) boundary->attach(solver);
+    }
+
     mutable device::Solver solver;
     std::optional<device::GenericRunner<RHSType>> generic; // models on the generic kernel
+    std::optional<device::BCRunner<typename PDEModelType::bc_t>> boundary; // custom boundary functors
 
   private:
     std::optional<PlotExporter> exporter;

@@ -265,11 +265,12 @@ struct BCType {
 class BCRec {
   public:
     BCRec() = default;
-    BCRec(int loX, int loY, int loZ, int hiX, int hiY, int hiZ) : bc{loX, loY, loZ, hiX, hiY, hiZ} {}
-    int lo(int d) const { return bc[d]; }
-    int hi(int d) const { return bc[3 + d]; }
-    void setLo(int d, int v) { bc[d] = v; }
-    void setHi(int d, int v) { bc[3 + d] = v; }
+    AMREX_GPU_HOST_DEVICE BCRec(int loX, int loY, int loZ, int hiX, int hiY, int hiZ)
+        : bc{loX, loY, loZ, hiX, hiY, hiZ} {}
+    AMREX_GPU_HOST_DEVICE int lo(int d) const { return bc[d]; }
+    AMREX_GPU_HOST_DEVICE int hi(int d) const { return bc[3 + d]; }
+    AMREX_GPU_HOST_DEVICE void setLo(int d, int v) { bc[d] = v; }
+    AMREX_GPU_HOST_DEVICE void setHi(int d, int v) { bc[3 + d] = v; }
 
   private:
     int bc[6] = {BCType::bogus, BCType::bogus, BCType::bogus, BCType::bogus, BCType::bogus, BCType::bogus};

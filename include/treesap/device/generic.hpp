@@ -39,13 +39,14 @@
 
 namespace treesap::device {
 
-// Array4 view of a register of the stage view (ghosts included)
+// Array4 view of a register of the stage view (ghosts included), indexed as
+// the domain is: its first valid cell is `lo`, the domain's smallEnd
 template <class T>
-__host__ __device__ inline amrex::Array4<T> register_view(T *origin, const tsp_stage_view &v) {
+__host__ __device__ inline amrex::Array4<T> register_view(T *origin, const tsp_stage_view &v, const amrex::IntVect &lo) {
     const int G = v.ghost;
     T *p = origin - (G + G * v.sj + G * v.sk);
-    return amrex::Array4<T>(p, v.sj, v.sk, v.sc, amrex::Dim3{-G, -G, -G},
-                            amrex::Dim3{v.n[0] + G, v.n[1] + G, v.n[2] + G}, v.ncomp);
+    return amrex::Array4<T>(p, v.sj, v.sk, v.sc, amrex::Dim3{lo[0] - G, lo[1] - G, lo[2] - G},
+                            amrex::Dim3{lo[0] + v.n[0] + G, lo[1] + v.n[1] + G, lo[2] + v.n[2] + G}, v.ncomp);
 }
 
 template <class RHS, int KIND> // KIND: 0 first RK4 stage, 1 middle, 2 last, 3 Euler
@@ -54,11 +55,13 @@ __global__ __launch_bounds__(256) void generic_stage_kernel(const RHS rhs, const
     constexpr int ncomp = RHS::ncomp;
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= v.n[0] || j >= v.n[1]) return;
-    const amrex::Array4<const amrex::Real> src = register_view<const amrex::Real>(v.src, v);
-    const amrex::Array4<const amrex::Real> init = register_view<const amrex::Real>(v.init, v);
-    const amrex::Array4<amrex::Real> acc = register_view<amrex::Real>(v.acc, v);
+    // cells carry the domain's own indices, as the reference's ParallelFor over validbox gives them
+    const amrex::IntVect lo = geom.Domain().smallEnd();
+    const amrex::Array4<const amrex::Real> src = register_view<const amrex::Real>(v.src, v, lo);
+    const amrex::Array4<const amrex::Real> init = register_view<const amrex::Real>(v.init, v, lo);
+    const amrex::Array4<amrex::Real> acc = register_view<amrex::Real>(v.acc, v, lo);
     const amrex::Real b = v.b, a_next = v.a_next, t = v.time;
-    const amrex::IntVect idx{i, j, k};
+    const amrex::IntVect idx{lo[0] + i, lo[1] + j, lo[2] + k};
     typename RHS::Workspace workspace{};
     amrex::Real kv[ncomp];
     treesap::static_for<ncomp>([&]<int comp>() { kv[comp] = rhs.template operator()<comp>(idx, t, src, geom, workspace); });
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(256) void generic_stage_kernel(const RHS rhs, const
             if constexpr (KIND == 0) acc(idx, comp) = init(idx, comp) + b * kv[comp];
             else acc(idx, comp) += b * kv[comp];
             if constexpr (KIND != 2) {
-                const amrex::Array4<amrex::Real> argn = register_view<amrex::Real>(v.argn, v);
+                const amrex::Array4<amrex::Real> argn = register_view<amrex::Real>(v.argn, v, lo);
                 argn(idx, comp) = init(idx, comp) + a_next * kv[comp];
             }
         }

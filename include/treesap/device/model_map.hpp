@@ -290,14 +290,18 @@ template <class In, class P> struct decorators<ABLDecorator<In, P>> {
 
 // ---- boundary functors -----------------------------------------------------
 
+// Any functor but the two the library carries is applied on the device by a
+// kernel compiled from it (device/generic_bc.hpp, through tsp_set_bc_callback).
 template <class BC> struct bc_map {
-    static_assert(detail::always_false<BC>,
-                  "treesap: the device library carries NoOpBC and DirichletBC as physical boundary functors");
+    static constexpr bool custom = true;
+    static void fill(const BC &, tsp_desc &d, bool) { d.bc = TSP_BC_CUSTOM; }
 };
 template <int nc, int ng> struct bc_map<NoOpBC<nc, ng>> {
+    static constexpr bool custom = false;
     static void fill(const NoOpBC<nc, ng> &, tsp_desc &d, bool) { d.bc = TSP_BC_NOOP; }
 };
 template <int nc, int ng> struct bc_map<DirichletBC<nc, ng>> {
+    static constexpr bool custom = false;
     static void fill(const DirichletBC<nc, ng> &b, tsp_desc &d, bool has_abl) {
         d.bc = TSP_BC_DIRICHLET;
         for (int c = 0; c < nc; ++c) {
@@ -313,6 +317,8 @@ template <int nc, int ng> struct bc_map<DirichletBC<nc, ng>> {
 
 // Whether a model runs on the library's kernels (else on the generic kernel).
 template <class Model> constexpr bool runs_library_kernels = is_mapped<typename decorators<typename Model::rhs_t>::user_t>;
+// Whether a model's boundary functor runs on a kernel compiled from it.
+template <class Model> constexpr bool custom_boundary = bc_map<typename Model::bc_t>::custom;
 
 template <class Model> tsp_desc make_desc(const Model &m) {
     static_assert(Model::ncomp <= TSP_MAX_COMP, "treesap: at most 12 components");

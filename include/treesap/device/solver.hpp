@@ -25,14 +25,16 @@ class Solver {
     Solver() = default;
     // `block`: the largest box per axis (SingleLevelIntegrator(model, block_size)); a
     // block smaller than the domain along any axis splits the level into boxes
-    // (the (phi, Pi), (phi, u) and NonLinear systems; the tensor system and
-    // functors on the generic kernel run one box, results being the same)
+    // (the (phi, Pi), (phi, u) and NonLinear systems; the tensor system,
+    // functors on the generic kernel and custom boundary functors run one
+    // box, results being the same)
     explicit Solver(const tsp_desc &d, const int *block = nullptr, void *hip_stream = nullptr) : desc_(d) {
         bool cut = false;
         if (block)
             for (int a = 0; a < 3; ++a) cut = cut || block[a] < d.n[a];
-        const bool level_system = d.system == TSP_SYS_WAVE_PHIPI || d.system == TSP_SYS_WAVE_PHIU ||
-                                  d.system == TSP_SYS_NONLINEAR;
+        const bool level_system = (d.system == TSP_SYS_WAVE_PHIPI || d.system == TSP_SYS_WAVE_PHIU ||
+                                   d.system == TSP_SYS_NONLINEAR) &&
+                                  d.bc != TSP_BC_CUSTOM;
         if (cut && level_system) {
             const int32_t blk[3] = {block[0], block[1], block[2]};
             check(tsp_level_create(&desc_, blk, hip_stream, &l_), "tsp_level_create");

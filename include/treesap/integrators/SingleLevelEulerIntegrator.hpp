@@ -11,6 +11,7 @@ template <typename PDEModelType>
     requires PDEModelConcept<PDEModelType>
 class SingleLevelEulerIntegrator : public SingleLevelIntegrator<PDEModelType> {
     using Base = SingleLevelIntegrator<PDEModelType>;
+    using Base::attach_boundary;
     using Base::generic;
     using Base::library_kernels;
     using Base::require_model_dt;
@@ -22,6 +23,7 @@ class SingleLevelEulerIntegrator : public SingleLevelIntegrator<PDEModelType> {
 
     amrex::Real computeNewState(amrex::Real delta_time) override {
         require_model_dt(delta_time);
+        attach_boundary();
         if constexpr (library_kernels) solver.euler_steps(1);
         else generic->euler_steps(solver, 1);
         return delta_time;
@@ -29,6 +31,7 @@ class SingleLevelEulerIntegrator : public SingleLevelIntegrator<PDEModelType> {
 
   protected:
     void device_steps(std::int64_t n, bool) override {
+        attach_boundary();
         if constexpr (library_kernels) solver.euler_steps(n);
         else generic->euler_steps(solver, n);
     }

@@ -16,6 +16,8 @@ template <typename PDEModelType>
     requires PDEModelConcept<PDEModelType>
 class SingleLevelRK4Integrator : public SingleLevelIntegrator<PDEModelType> {
     using Base = SingleLevelIntegrator<PDEModelType>;
+    using Base::attach_boundary;
+    using Base::custom_boundary;
     using Base::generic;
     using Base::library_kernels;
     using Base::require_model_dt;
@@ -28,15 +30,19 @@ class SingleLevelRK4Integrator : public SingleLevelIntegrator<PDEModelType> {
     // one RK4 step of the whole level
     amrex::Real computeNewState(amrex::Real delta_time) override {
         require_model_dt(delta_time);
+        attach_boundary();
         if constexpr (library_kernels) solver.rk4_steps(1);
         else generic->rk4_steps(solver, 1);
         return delta_time;
     }
 
   protected:
+    // graph replay where it applies: not under a custom boundary functor, whose
+    // fills each pass their own time
     void device_steps(std::int64_t n, bool graph) override {
+        attach_boundary();
         if constexpr (!library_kernels) generic->rk4_steps(solver, n);
-        else if (graph) solver.graph_rk4_steps(n);
+        else if (graph && !custom_boundary) solver.graph_rk4_steps(n);
         else solver.rk4_steps(n);
     }
 };

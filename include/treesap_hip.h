@@ -58,8 +58,10 @@ typedef enum tsp_system {
 
 /* Physical boundary functor on non-periodic faces (BCConcept,
  * PDEModelConcepts.hpp:70-90): NoOpBC (bcs/NoOpBC.hpp:22-47) leaves ghost
- * memory untouched; DirichletBC (bcs/DirichletBC.hpp:21-66) writes target[]. */
-typedef enum tsp_bc { TSP_BC_NOOP = 0, TSP_BC_DIRICHLET = 1 } tsp_bc;
+ * memory untouched; DirichletBC (bcs/DirichletBC.hpp:21-66) writes target[];
+ * TSP_BC_CUSTOM: any other functor, applied by the caller's callback
+ * (tsp_set_bc_callback) after the library's periodic fill. */
+typedef enum tsp_bc { TSP_BC_NOOP = 0, TSP_BC_DIRICHLET = 1, TSP_BC_CUSTOM = 2 } tsp_bc;
 
 /* ABLProfileConcept (rhs-decorators/ABLDecorator.hpp:24-29) instances. */
 typedef enum tsp_profile { TSP_PROFILE_LINEAR = 0, TSP_PROFILE_QUADRATIC = 1, TSP_PROFILE_CUBIC = 2 } tsp_profile;
@@ -185,8 +187,38 @@ tsp_status tsp_init_expr(tsp_solver *s, const tsp_expr_op *ops, const int32_t *c
 
 /* Replaces SingleLevelIntegrator::FillPatch (SingleLevelIntegrator.hpp:153-163)
  * on the register with role `role` (tsp_role): periodic images then the
- * physical-boundary functor, faces, edges and corners. */
+ * physical-boundary functor, faces, edges and corners (a TSP_BC_CUSTOM
+ * functor at the solver's current time). */
 tsp_status tsp_fill_ghosts(tsp_solver *s, int32_t role);
+
+/* Replaces the physical-boundary step of FillPatch for a boundary functor the
+ * library does not carry (TSP_BC_CUSTOM): the amrex::PhysBCFunct over
+ * amrex::GpuBndryFuncFab<BCType>(model.get_bc()) that SingleLevelIntegrator.hpp:153-163
+ * builds. After each fill's periodic step the library calls `fn` on its
+ * stream with the register just filled and the fill's time: `time` before a
+ * step (pre_step, SingleLevelIntegrator.hpp:102-108), time + dt * a_fraction[s]
+ * before RK4 stage s >= 1 (integrators/SingleLevelRK4Integrator.hpp:119-125).
+ * The callback launches the caller's kernel, which applies the functor to
+ * every ghost cell outside the domain along a non-periodic axis (faces,
+ * edges and corners); those cells hold, on entry, what the register last
+ * held there. `fn` returns 0 on success; any other value stops the step with
+ * TSP_HIP_ERROR. A TSP_BC_CUSTOM solver steps only after a callback is set
+ * (TSP_INVALID_ARG otherwise), reads ghost memory (ghost_mode 1; stage pairs
+ * and ghost_mode 0 are refused), steps eagerly (graph replay, which would
+ * freeze the fill times, returns TSP_UNSUPPORTED), and cannot join a batch,
+ * a level or an adjoint solve. */
+typedef struct tsp_bc_view {
+    int32_t n[3];        /* valid cells per axis */
+    int32_t ghost;       /* ghost width G */
+    int32_t ncomp;
+    int32_t periodic[3];
+    double time;         /* the fill's time */
+    double *reg;         /* the register filled: element (i,j,k,c), ghosts included
+                            (-G <= i < n + G), at reg[i + j*sj + k*sk + c*sc] */
+    int64_t sj, sk, sc;
+} tsp_bc_view;
+typedef int (*tsp_bc_fn)(void *user, const tsp_bc_view *view, void *hip_stream);
+tsp_status tsp_set_bc_callback(tsp_solver *s, tsp_bc_fn fn, void *user);
 /* Copy a whole ghosted register (or its valid region) to host, for tests. */
 tsp_status tsp_get_register(tsp_solver *s, int32_t role, double *dst_valid_plus_ghost);
 

@@ -6,10 +6,12 @@ dropin_gpu   TreESaP-style programs run through the drop-in integrators on the
              GPU, their right-hand sides mapped onto the library's kernels;
 dropin_generic  programs whose right-hand sides the library does not carry,
              run on the stage kernel compiled from the functor;
+dropin_bc    programs whose boundary functor the library does not carry, run
+             on the boundary kernel compiled from the functor;
 generic_timing  the throughput of that stage kernel on a (phi, Pi) functor
              (bench.py's generic leg).
 
-The first two are ordinary g++ C++20 programs, the last two are compiled by hipcc
+The first two are ordinary g++ C++20 programs, the last three are compiled by hipcc
 for gfx950; all include include/treesap/ and link
 treesap_amd/libtreesap_hip.so (found at run time through an $ORIGIN rpath).
 Usage: python tests/cpp/build.py
@@ -25,9 +27,9 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 BIN = HERE / "bin"
-PROGRAMS = ("dropin_host", "dropin_gpu", "dropin_generic", "generic_timing")
+PROGRAMS = ("dropin_host", "dropin_gpu", "dropin_generic", "dropin_bc", "generic_timing")
 # programs whose right-hand sides run on the stage kernel compiled from the functor
-HIP_PROGRAMS = ("dropin_generic", "generic_timing")
+HIP_PROGRAMS = ("dropin_generic", "dropin_bc", "generic_timing")
 
 
 def _hipcc() -> str:

@@ -187,3 +187,21 @@ def test_level_validation():
     d2 = make_desc(Case(system="tensor", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01))
     assert L.tsp_level_create(C.byref(d2), (C.c_int32 * 3)(8, 8, 8), None, C.byref(h)) == _lib.UNSUPPORTED
     assert "(phi,Pi), (phi,u) and NonLinear" in L.tsp_last_error().decode()
+
+
+def test_custom_boundary_validation():
+    """A boundary functor the library does not carry (TSP_BC_CUSTOM) is a valid descriptor, but
+    levels and batches, whose exchange kernels carry NoOp and Dirichlet boundaries only, refuse
+    it before any HIP call; an unknown bc id stays invalid."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    d = make_desc(Case(system="phipi", n=(16, 16, 16), stencil_n=2, ghost=2, ko_order=2, dt=0.01))
+    d.bc = 2
+    assert L.tsp_level_create(C.byref(d), (C.c_int32 * 3)(8, 8, 8), None, C.byref(h)) == _lib.UNSUPPORTED
+    assert "NoOp and Dirichlet boundaries only" in L.tsp_last_error().decode()
+    assert L.tsp_batch_create(C.byref(d), 1, None, C.byref(h)) == _lib.UNSUPPORTED
+    assert "NoOp and Dirichlet boundaries only" in L.tsp_last_error().decode()
+    d.bc = 3
+    assert L.tsp_level_create(C.byref(d), (C.c_int32 * 3)(8, 8, 8), None, C.byref(h)) == _lib.INVALID_ARG
+    assert "unknown bc id" in L.tsp_last_error().decode()
+    assert L.tsp_set_bc_callback(None, None, None) == _lib.INVALID_ARG

@@ -524,3 +524,39 @@ def test_generic_kernel_program(generic_runs, scenario):
         want = np.fromfile(d / "host.bin", dtype=np.float64)
     assert np.isfinite(final).all() and not np.array_equal(final, init)
     assert np.array_equal(final, want), f"{scenario}: max abs diff {np.max(np.abs(final - want))}"
+
+
+# ---- boundary functors the library does not carry (tests/cpp/dropin_bc.cpp) ----
+
+BC_SCENARIOS = {"library_rk4": 0, "library_euler": 0, "tensor": 3, "generic_shifted": 4}  # -> descriptor system
+
+
+@pytest.fixture(scope="module")
+def bc_runs(tmp_path_factory):
+    out = tmp_path_factory.mktemp("custom_bc")
+    print(f"[dropin_bc] scenarios {list(BC_SCENARIOS)} -> {out}", flush=True)
+    _run([cpp_build.binary("dropin_bc"), out, *BC_SCENARIOS])
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scenario", list(BC_SCENARIOS))
+def test_custom_boundary_program(bc_runs, scenario):
+    """A boundary functor other than NoOpBC and DirichletBC (time- and position-dependent
+    values, copies of the nearest valid cell scaled by the model's BCRec) runs on the kernel
+    compiled from it after each fill's periodic step (descriptor bc TSP_BC_CUSTOM), under the
+    library's (phi, Pi) and tensor kernels (RK4 with autotuning and advance(), Euler) and the
+    generic kernel on a domain whose first cell is not (0, 0, 0) with an absorbing layer. The
+    result matches the same functors stepped on the host (tests/cpp/host_rk4.hpp) bitwise, and
+    differs from the NoOp boundary's."""
+    d = bc_runs / scenario
+    kind, steps, time, system, bc = (d / "meta.txt").read_text().split()
+    assert int(bc) == 2, "the custom boundary path did not run"
+    assert int(system) == BC_SCENARIOS[scenario]
+    final = np.fromfile(d / "final.bin", dtype=np.float64)
+    init = np.fromfile(d / "init.bin", dtype=np.float64)
+    want = np.fromfile(d / "host.bin", dtype=np.float64)
+    noop = np.fromfile(d / "noop.bin", dtype=np.float64)
+    assert np.isfinite(final).all() and not np.array_equal(final, init)
+    assert not np.array_equal(want, noop), "the boundary functor does not affect this scenario"
+    assert np.array_equal(final, want), f"{scenario}: max abs diff {np.max(np.abs(final - want))}"

@@ -167,6 +167,7 @@ PROTOTYPES = {
     # stage callback: int (*)(void *user, const tsp_stage_view *view, void *hip_stream)
     "tsp_rk4_steps_custom": (C.c_int, [_VP, C.c_int64, _VP, _VP]),
     "tsp_euler_steps_custom": (C.c_int, [_VP, C.c_int64, _VP, _VP]),
+    "tsp_set_bc_callback": (C.c_int, [_VP, _VP, _VP]),
     "tsp_batch_adjoint_create": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(_VP)]),
     "tsp_batch_adjoint_destroy": (C.c_int, [_VP]),

@@ -311,6 +311,9 @@ tsp_status create_batch(const tsp_desc *descs, int32_t count, void *stream, tsp_
     for (int i = 0; i < count; ++i) {
         tsp_status st = box_off ? TSP_OK : validate(&descs[i]);
         if (st != TSP_OK) return fail(st, "instance " + std::to_string(i) + ": " + g_err);
+        if (descs[i].bc == TSP_BC_CUSTOM)
+            return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) +
+                                             ": batches and levels carry NoOp and Dirichlet boundaries only");
         if (tsp::batch_system_slot(descs[i].system) < 0)
             return fail(TSP_UNSUPPORTED, "instance " + std::to_string(i) +
                                              ": batched mode runs the (phi,Pi), (phi,u) and NonLinear systems");
@@ -865,6 +868,7 @@ tsp_status tsp_level_create(const tsp_desc *d, const int32_t *block, void *strea
     if (!block) return fail(TSP_INVALID_ARG, "block size is NULL");
     tsp_status st = validate(d);
     if (st != TSP_OK) return st;
+    if (d->bc == TSP_BC_CUSTOM) return fail(TSP_UNSUPPORTED, "multi-box levels carry NoOp and Dirichlet boundaries only");
     if (tsp::batch_system_slot(d->system) < 0)
         return fail(TSP_UNSUPPORTED, "multi-box levels run the (phi,Pi), (phi,u) and NonLinear systems");
     for (int a = 0; a < 3; ++a)

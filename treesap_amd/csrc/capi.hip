@@ -50,6 +50,9 @@ struct tsp_solver {
     // tensor system with KO: the K components' KO sums of the current stage
     // (six components in the register layout, written and read by the stage launch)
     double *kko = nullptr;
+    // TSP_BC_CUSTOM: the caller's physical-boundary callback, run after each periodic fill
+    tsp_bc_fn bc_fn = nullptr;
+    void *bc_user = nullptr;
     tsp::LaunchFn launch = nullptr;
     tsp::LaunchFn launch_stage[4] = {nullptr, nullptr, nullptr, nullptr}; // stage-specific, where built
     tsp::PairFn launch_pair = nullptr;                                      // stage pairs, where built
@@ -74,8 +77,9 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     // the tensor kernel: 16 x 8 x 2 blocks, block order in slabs of two z tiles
     if (d.system == TSP_SYS_TENSOR_ADM) return tsp_kernel_cfg{1, 8, 2, 1, 0, 0, 0};
     tsp_kernel_cfg c{0, d.n[1] >= 16 ? 16 : 8, 64, 1, 1, 1, 0};
-    // mixed stencils read filled edge/corner ghosts; a custom kernel reads ghost memory
-    if (d.system == TSP_SYS_TENSOR_ADM || d.system == TSP_SYS_CUSTOM) c.ghost_mode = 1;
+    // mixed stencils read filled edge/corner ghosts; a custom kernel reads ghost
+    // memory, and so does every kernel under a custom boundary functor
+    if (d.system == TSP_SYS_TENSOR_ADM || d.system == TSP_SYS_CUSTOM || d.bc == TSP_BC_CUSTOM) c.ghost_mode = 1;
     const long long tiles = (long long)((d.n[0] + 63) / 64) * ((d.n[1] + c.tile_y - 1) / c.tile_y);
     for (int zc : {64, 32, 16, 8}) {
         c.zchunk = std::min(zc, d.n[2]);
@@ -87,7 +91,8 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
 // The tensor system's stage kernel reads ghost memory (edges and corners for
 // its mixed stencils), so it always runs with the fill pass.
 bool fills_ghosts(const tsp_solver *s) {
-    return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM || s->d.system == TSP_SYS_CUSTOM;
+    return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM || s->d.system == TSP_SYS_CUSTOM ||
+           s->d.bc == TSP_BC_CUSTOM;
 }
 
 double *reg_origin(const tsp_solver *s, int reg) {
@@ -116,6 +121,7 @@ bool pairs_apply(const tsp_solver *s, int ghost_mode, std::string *why) {
     };
     if (!s->launch_pair) return no("stage_pairs needs the (phi, Pi) system with KO dissipation");
     if (ghost_mode != 0) return no("stage_pairs needs ghost_mode 0");
+    if (s->d.bc == TSP_BC_CUSTOM) return no("stage_pairs needs a boundary functor the library carries");
     if (s->ptr_access || (int64_t)sizeof(double) * s->reg_elems >= (int64_t(1) << 32))
         return no("stage_pairs needs registers under 4 GiB");
     for (int a = 0; a < 3; ++a)
@@ -193,6 +199,30 @@ tsp::StageArgs stage_args(const tsp_solver *s, int stage, int src_reg, int init_
     return a;
 }
 
+// FillPatch (SingleLevelIntegrator.hpp:153-163) of register `reg` at time
+// `time`: the library's periodic (and Dirichlet) fill, then, under a custom
+// boundary functor, the caller's callback on the physical-boundary ghosts.
+tsp_status fill_patch(tsp_solver *s, int reg, double time) {
+    TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, reg), s->stream));
+    if (s->d.bc != TSP_BC_CUSTOM) return TSP_OK;
+    if (!s->bc_fn) return fail(TSP_INVALID_ARG, "a TSP_BC_CUSTOM solver steps only after tsp_set_bc_callback");
+    tsp_bc_view v{};
+    for (int a = 0; a < 3; ++a) {
+        v.n[a] = s->d.n[a];
+        v.periodic[a] = s->d.periodic[a];
+    }
+    v.ghost = s->G;
+    v.ncomp = s->nc;
+    v.time = time;
+    v.reg = reg_origin(s, reg);
+    v.sj = s->sj;
+    v.sk = s->sk;
+    v.sc = s->sc;
+    const int r = s->bc_fn(s->bc_user, &v, (void *)s->stream);
+    if (r != 0) return fail(TSP_HIP_ERROR, "boundary callback failed (returned " + std::to_string(r) + ")");
+    return TSP_OK;
+}
+
 struct EventPair {
     hipEvent_t start = nullptr, stop = nullptr;
 };
@@ -211,10 +241,11 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
     const double dt = s->d.dt;
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
         &ARGN = s->role[TSP_ROLE_ARG_NEXT];
+    // slot 0: pre_step at `time`; slot s: before stage s at time + dt * a_fraction[s]
     auto fill = [&](int reg, int slot) -> tsp_status {
         if (!fills_ghosts(s)) return TSP_OK;
         if (tm) TSP_HIP(hipEventRecord(tm->fill[slot].start, s->stream));
-        TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, reg), s->stream));
+        if (tsp_status f = fill_patch(s, reg, s->time + dt * kAFrac[slot]); f != TSP_OK) return f;
         if (tm) TSP_HIP(hipEventRecord(tm->fill[slot].stop, s->stream));
         return TSP_OK;
     };
@@ -277,7 +308,8 @@ tsp_status enqueue_rk4_step(tsp_solver *s, StepTiming *tm) {
 tsp_status enqueue_euler_step(tsp_solver *s) {
     if (!s->launch) return fail(TSP_UNSUPPORTED, "a custom model steps through tsp_euler_steps_custom");
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
-    if (fills_ghosts(s)) TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
+    if (fills_ghosts(s))
+        if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f;
     const tsp::StageArgs a = stage_args(s, tsp::kEuler, OLD, OLD, NEW, -1, s->d.dt, 0.0);
     TSP_HIP(s->launch(a, s->cfg.tile_y, s->stream));
     std::swap(OLD, NEW);
@@ -292,6 +324,9 @@ void advance_clock(tsp_solver *s) { // Integrator.hpp:98-100
 // The captured two-step graph for register roles `roles` (capturing it if
 // missing); graphs of another kernel configuration are dropped first.
 tsp_status ensure_graph(tsp_solver *s, const int roles[4], int *slot) {
+    if (s->d.bc == TSP_BC_CUSTOM)
+        return fail(TSP_UNSUPPORTED, "a custom boundary functor is called with each fill's time; graph replay "
+                                     "would freeze those times (step eagerly)");
     if (std::memcmp(&s->graph_cfg, &s->cfg, sizeof(tsp_kernel_cfg)) != 0) {
         drop_graphs(s);
         s->graph_cfg = s->cfg;
@@ -532,7 +567,14 @@ tsp_status tsp_init_expr(tsp_solver *s, const tsp_expr_op *ops, const int32_t *c
 tsp_status tsp_fill_ghosts(tsp_solver *s, int32_t role) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     if (role < 0 || role > 3) return fail(TSP_INVALID_ARG, "role must be 0..3");
-    TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, s->role[role]), s->stream));
+    return fill_patch(s, s->role[role], s->time);
+}
+
+tsp_status tsp_set_bc_callback(tsp_solver *s, tsp_bc_fn fn, void *user) {
+    if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
+    if (s->d.bc != TSP_BC_CUSTOM) return fail(TSP_INVALID_ARG, "the solver's boundary functor is not TSP_BC_CUSTOM");
+    s->bc_fn = fn;
+    s->bc_user = user;
     return TSP_OK;
 }
 
@@ -614,6 +656,7 @@ tsp_status tsp_rk4_steps_noalloc(tsp_solver *s, const double *state_in, double *
     for (int64_t n = 0; n < nsteps; ++n) {
         st = enqueue_rk4_step(&t, nullptr);
         if (st != TSP_OK) return st;
+        advance_clock(&t); // the copy's clock: the fill times of a custom boundary functor
     }
     TSP_HIP(tsp::launch_pack(state_out, reg_origin(&t, t.role[TSP_ROLE_OLD]), s->d.n[0], s->d.n[1], s->d.n[2], s->nc,
                              (int)s->sj, (int)s->sk, (int)s->sc, s->stream));
@@ -677,7 +720,7 @@ tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, 
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
         &ARGN = s->role[TSP_ROLE_ARG_NEXT];
     for (int64_t n = 0; n < nsteps; ++n) {
-        TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream)); // pre_step FillPatch
+        if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f; // pre_step FillPatch
         for (int stage = 0; stage < 4; ++stage) {
             // integrators/SingleLevelRK4Integrator.hpp:115-125
             const double a_next = dt * kAFrac[(stage + 1) % 4];
@@ -685,7 +728,7 @@ tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, 
             const double b = dt * kBFrac[stage];
             if (stage > 0) {
                 std::swap(ARG, ARGN);
-                TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, ARG), s->stream));
+                if (tsp_status f = fill_patch(s, ARG, time_arg); f != TSP_OK) return f;
             }
             const tsp_stage_view v =
                 stage_view(s, stage, stage == 0 ? OLD : ARG, OLD, NEW, stage < 3 ? ARGN : -1, time_arg, b, a_next);
@@ -703,7 +746,7 @@ tsp_status tsp_euler_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
     for (int64_t n = 0; n < nsteps; ++n) {
-        TSP_HIP(tsp::launch_ghost_fill(ghost_args(s, OLD), s->stream));
+        if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f;
         const tsp_stage_view v = stage_view(s, tsp::kEuler, OLD, OLD, NEW, -1, s->time, s->d.dt, 0.0);
         tsp_status st = call_stage(s, fn, user, v);
         if (st != TSP_OK) return st;
@@ -732,6 +775,8 @@ tsp_status tsp_set_kernel(tsp_solver *s, const tsp_kernel_cfg *cfg) {
     if (st != TSP_OK) return st;
     std::string why;
     if (cfg->stage_pairs && !pairs_apply(s, cfg->ghost_mode, &why)) return fail(TSP_UNSUPPORTED, why);
+    if (s->d.bc == TSP_BC_CUSTOM && cfg->ghost_mode != 1)
+        return fail(TSP_INVALID_ARG, "a custom boundary functor needs ghost_mode 1 (the fill pass)");
     s->cfg = *cfg;
     return TSP_OK;
 }
@@ -819,6 +864,9 @@ tsp_status tsp_register_view(tsp_solver *s, int32_t role, double **ptr, int64_t 
 tsp_status tsp_graph_rk4_steps(tsp_solver *s, int64_t nsteps) {
     if (!s) return fail(TSP_INVALID_ARG, "solver is NULL");
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
+    if (s->d.bc == TSP_BC_CUSTOM)
+        return fail(TSP_UNSUPPORTED, "a custom boundary functor is called with each fill's time; graph replay "
+                                     "would freeze those times (step eagerly)");
     int slot = -1;
     if (nsteps >= 2) {
         tsp_status st = ensure_graph(s, s->role, &slot);
@@ -954,7 +1002,8 @@ bool cache_lookup(const tsp_solver *s, const std::string &key, tsp_kernel_cfg *c
         if (count_fields(line) == 9 &&
             std::sscanf(line, "%255s %d %d %d %d %d %d %d %lf", k, &c.ghost_mode, &c.tile_y, &c.zchunk, &c.xcd_remap,
                         &c.zalt, &c.halo_ring, &c.stage_pairs, &ms) == 9 &&
-            key == k && validate_cfg(&c) == TSP_OK && (!c.stage_pairs || pairs_apply(s, c.ghost_mode, nullptr))) {
+            key == k && validate_cfg(&c) == TSP_OK && (!c.stage_pairs || pairs_apply(s, c.ghost_mode, nullptr)) &&
+            (s->d.bc != TSP_BC_CUSTOM || c.ghost_mode == 1)) {
             *cfg = c;
             found = true;
         }
@@ -1017,12 +1066,14 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
         for (int ty : {8, 4, 16})
             for (int zc : {1, 2, 4}) cands.push_back(tsp_kernel_cfg{1, ty, zc, 1, 0, 0, 0});
     } else {
+        // a custom boundary functor needs the fill pass (ghost_mode 1) in every candidate
+        const int gm = s->d.bc == TSP_BC_CUSTOM ? 1 : 0;
         for (int ty : {2, 4, 8, 16, 32})
             for (int zc : zchunks)
                 for (int xr = 0; xr < 2; ++xr)
                     for (int za = 0; za < 3; ++za)
                         for (int hr = 0; hr < (ty >= 8 ? 2 : 1); ++hr)
-                            cands.push_back(tsp_kernel_cfg{0, ty, zc, xr, za, hr, 0});
+                            cands.push_back(tsp_kernel_cfg{gm, ty, zc, xr, za, hr, 0});
     }
     // stage pairs (32 x 32 tiles; tile_y, zalt and halo_ring do not apply)
     if (pairs_apply(s, 0, nullptr))
@@ -1403,6 +1454,8 @@ tsp_status tsp_adjoint_create(tsp_solver *s, const int32_t *receivers, int32_t n
         return fail(TSP_UNSUPPORTED, "the discrete adjoint covers the (phi,Pi) and NonLinear systems");
     if (s->d.system == TSP_SYS_NONLINEAR && (s->d.stencil_n < 1 || s->d.stencil_n > 4))
         return fail(TSP_UNSUPPORTED, "the NonLinear adjoint needs stencil_n in 1..4");
+    if (s->d.bc == TSP_BC_CUSTOM)
+        return fail(TSP_UNSUPPORTED, "the discrete adjoint transposes the NoOp and Dirichlet boundaries only");
     if (nrec < 1 || !receivers) return fail(TSP_INVALID_ARG, "need at least one receiver");
     if (nsteps < 1) return fail(TSP_INVALID_ARG, "nsteps must be >= 1");
     if (checkpoint_every < 1) return fail(TSP_INVALID_ARG, "checkpoint_every must be >= 1");

@@ -161,7 +161,8 @@ inline tsp_status validate(const tsp_desc *d) {
         if (d->periodic[a] && d->ghost > d->n[a])
             return fail(TSP_INVALID_ARG, "periodic axis needs n >= ghost width");
     }
-    if (d->bc != TSP_BC_NOOP && d->bc != TSP_BC_DIRICHLET) return fail(TSP_INVALID_ARG, "unknown bc id");
+    if (d->bc != TSP_BC_NOOP && d->bc != TSP_BC_DIRICHLET && d->bc != TSP_BC_CUSTOM)
+        return fail(TSP_INVALID_ARG, "unknown bc id");
     if (d->abl_thickness < 0) return fail(TSP_INVALID_ARG, "abl_thickness must be >= 0");
     if (d->abl_thickness > 0) {
         for (int a = 0; a < 3; ++a)
