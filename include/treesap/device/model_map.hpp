@@ -103,8 +103,6 @@ struct Recognised {
 
 namespace detail {
 
-template <class> constexpr bool always_false = false;
-
 // SplitMix64 (Steele, Lea, Flood 2014), uniform doubles in [-1, 1).
 struct SplitMix64 {
     std::uint64_t s;
