@@ -7,12 +7,16 @@
 //   host.bin   the same steps of the same functors on the host (host_rk4.hpp)
 //   noop.bin   the host steps with ghost memory left untouched instead (NoOpBC),
 //              to show the boundary functor matters
+//   noalloc.bin, noalloc_host.bin (library_rk4): two more steps from the final
+//              state through tsp_rk4_steps_noalloc, and the same on the host
 //   meta.txt   "<integrator> <steps> <time as %.17g> <system id> <bc id>"
 //
 //   dropin_bc <outdir> [scenario ...]
 #include "treesap.hpp"
 #include "host_rk4.hpp"
 #include "user_rhs_traits.hpp"
+
+#include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <filesystem>
@@ -140,6 +144,32 @@ static void scenario_library(const fs::path &dir, bool euler) {
         integ.advance(3); // eager steps: each fill passes its own time
         dump(dir, integ, u0, "rk4");
         steps = integ.get_steps();
+        // two more steps through the allocation-free entry point (the integrator's
+        // boundary callback stays attached), from the final state and time
+        const std::vector<double> s = integ.get_state();
+        const std::size_t bytes = s.size() * sizeof(double);
+        double *din = nullptr, *dout = nullptr;
+        if (hipMalloc((void **)&din, bytes) != hipSuccess || hipMalloc((void **)&dout, bytes) != hipSuccess)
+            throw std::runtime_error("hipMalloc failed");
+        if (hipMemcpy(din, s.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+        device::check(tsp_rk4_steps_noalloc(integ.device_solver().handle(), din, dout, 2), "tsp_rk4_steps_noalloc");
+        integ.device_solver().sync(); // the steps are enqueued on the solver's (non-blocking) stream
+        std::vector<double> na(s.size());
+        if (hipMemcpy(na.data(), dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+        (void)hipFree(din);
+        (void)hipFree(dout);
+        write_bin(dir / "noalloc.bin", na);
+        HostStepper<RHSType> host(rhs, geom, dt, false, nullptr);
+        const amrex::GeometryData gd = geom.data();
+        host.custom_bc = [bc, gd, recs](const amrex::IntVect &iv, const amrex::Array4<double> &a, double tt) {
+            bc(iv, a, 0, RHSType::ncomp, gd, tt, recs.data(), 0, 0);
+        };
+        host.time = integ.get_time();
+        host.set_state(s);
+        for (int q = 0; q < 2; ++q) host.rk4_step();
+        write_bin(dir / "noalloc_host.bin", host.get_state());
     }
     write_bin(dir / "host.bin", host_steps(rhs, bc, geom, dt, recs, u0, steps, euler));
     write_bin(dir / "noop.bin", host_steps(rhs, bc, geom, dt, recs, u0, steps, euler, false));

@@ -548,7 +548,8 @@ def test_custom_boundary_program(bc_runs, scenario):
     library's (phi, Pi) and tensor kernels (RK4 with autotuning and advance(), Euler) and the
     generic kernel on a domain whose first cell is not (0, 0, 0) with an absorbing layer. The
     result matches the same functors stepped on the host (tests/cpp/host_rk4.hpp) bitwise, and
-    differs from the NoOp boundary's."""
+    differs from the NoOp boundary's. Two further steps through tsp_rk4_steps_noalloc under the
+    same boundary callback match the host too."""
     d = bc_runs / scenario
     kind, steps, time, system, bc = (d / "meta.txt").read_text().split()
     assert int(bc) == 2, "the custom boundary path did not run"
@@ -560,3 +561,8 @@ def test_custom_boundary_program(bc_runs, scenario):
     assert np.isfinite(final).all() and not np.array_equal(final, init)
     assert not np.array_equal(want, noop), "the boundary functor does not affect this scenario"
     assert np.array_equal(final, want), f"{scenario}: max abs diff {np.max(np.abs(final - want))}"
+    if scenario == "library_rk4":
+        na = np.fromfile(d / "noalloc.bin", dtype=np.float64)
+        na_host = np.fromfile(d / "noalloc_host.bin", dtype=np.float64)
+        assert not np.array_equal(na, final)
+        assert np.array_equal(na, na_host), f"noalloc: max abs diff {np.max(np.abs(na - na_host))}"
