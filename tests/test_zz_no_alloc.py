@@ -81,6 +81,20 @@ def test_noalloc_with_stage_pairs_and_fill_mode():
         s.close()
 
 
+def test_noalloc_tensor_system():
+    """The tensor system's stage (its K components' KO sums go through the solver's own KO
+    buffer, allocated with the solver) under the allocation-free entry point, RK4."""
+    pytest.importorskip("torch")
+    case, st = small_case("tensor", n=(21, 18, 16), stencil_n=2, ko_order=2, periodic=(0, 1, 0), abl=3, seed=35)
+    _say(f"[noalloc] tensor {case.n}")
+    s = Solver(case)
+    got = _noalloc(s, st, 3)
+    want = _reference(case, st, 3)
+    assert np.isfinite(got).all() and np.array_equal(got, want)
+    assert s.steps == 0
+    s.close()
+
+
 def test_noalloc_rejects_too_large_solvers():
     torch = pytest.importorskip("torch")
     case, _ = small_case("phipi", n=(120, 120, 120), stencil_n=2, ko_order=2, seed=34)
