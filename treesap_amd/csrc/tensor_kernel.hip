@@ -223,13 +223,14 @@ template <int N, int R, int BX, int BY, int BZ> struct TensorTile {
             for (int m = 0; m < RPT; ++m)
                 if (hmask >> m & 1u) buf[tid + m * NT] = from[m];
         };
-        // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154)
+        // stage update of component c with right-hand side kv (SingleLevelRK4Integrator.hpp:142-154);
+        // the stores stream past the caches (non-temporal) that hold the regions' halos
         auto update = [&](int c, double kv, double cenc, double initv, double accv) {
             const long long o = off + (long long)c * a.sc;
             const double initc = read_init ? initv : cenc;
-            if (is_first) a.acc[o] = initc + a.b * kv;
-            else a.acc[o] = accv + a.b * kv;
-            if (write_argn) a.argn[o] = initc + a.a_next * kv;
+            if (is_first) __builtin_nontemporal_store(initc + a.b * kv, &a.acc[o]);
+            else __builtin_nontemporal_store(accv + a.b * kv, &a.acc[o]);
+            if (write_argn) __builtin_nontemporal_store(initc + a.a_next * kv, &a.argn[o]);
         };
         auto decorate = [&](int c, double kv, double ko_term, double cenc) {
             if constexpr (R > 0) kv = kv + a.ko_strength * ko_term;
@@ -470,7 +471,7 @@ template <int R, int KY, int KG> struct TensorKO {
                     sum += kx;
                     sum += ky;
                     sum += kz;
-                    if (active) a.kko[i + (long long)j * a.sj + z * sk + (long long)(g0 + g) * a.sc] = sum;
+                    if (active) __builtin_nontemporal_store(sum, &a.kko[i + (long long)j * a.sj + z * sk + (long long)(g0 + g) * a.sc]);
                 }
 #pragma unroll
                 for (int g = 0; g < KG; ++g) {
