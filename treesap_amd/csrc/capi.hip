@@ -88,11 +88,17 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     return c;
 }
 
-// The tensor system's stage kernel reads ghost memory (edges and corners for
-// its mixed stencils), so it always runs with the fill pass.
+// Whether a step runs the fill pass before its stages. A custom boundary
+// functor or stage kernel always does. The tensor system's stage kernel reads
+// ghost memory (edges and corners for its mixed stencils): it needs the pass
+// for periodic images; without a periodic axis every ghost cell lies outside a
+// non-periodic face and holds what the pass would write there (the Dirichlet
+// values written at creation, or, with NoOp, memory no stage writes). The
+// other kernels need it in ghost_mode 1.
 bool fills_ghosts(const tsp_solver *s) {
-    return s->cfg.ghost_mode == 1 || s->d.system == TSP_SYS_TENSOR_ADM || s->d.system == TSP_SYS_CUSTOM ||
-           s->d.bc == TSP_BC_CUSTOM;
+    if (s->d.system == TSP_SYS_CUSTOM || s->d.bc == TSP_BC_CUSTOM) return true;
+    if (s->d.system == TSP_SYS_TENSOR_ADM) return s->d.periodic[0] || s->d.periodic[1] || s->d.periodic[2];
+    return s->cfg.ghost_mode == 1;
 }
 
 double *reg_origin(const tsp_solver *s, int reg) {
@@ -1126,7 +1132,7 @@ tsp_status tsp_autotune(tsp_solver *s, int32_t steps, tsp_kernel_cfg *chosen) {
             best_ms = fine[bi];
         }
     }
-    if (st == TSP_OK && !tensor) { // (the tensor kernel always runs after the fill pass)
+    if (st == TSP_OK && !tensor) { // (the tensor kernel has no virtual-ghost mode to compare with)
         tsp_kernel_cfg fill = best;
         fill.ghost_mode = 1;
         fill.stage_pairs = 0;
