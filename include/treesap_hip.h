@@ -465,7 +465,10 @@ tsp_status tsp_batch_adjoint_backward(tsp_batch_adjoint *a, const double *dtrace
 
 /* Stepping of a model whose right-hand side the library does not carry
  * (TSP_SYS_CUSTOM): the library owns the registers, the FillPatch before every
- * stage (periodic images, then the Dirichlet/NoOp boundary values), the
+ * stage (periodic images, then the Dirichlet/NoOp boundary values; without a
+ * periodic axis and under NoOp/Dirichlet the ghost cells already hold those
+ * values and the pass is skipped, so the stage kernel writes valid cells
+ * only, as runStage does), the
  * register roles and the clock, exactly as tsp_rk4_steps / tsp_euler_steps do,
  * and calls `fn` once per stage to launch the caller's stage kernel on
  * `hip_stream` (SingleLevelRK4Integrator.hpp:110-157 runStage,

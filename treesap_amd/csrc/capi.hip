@@ -88,16 +88,17 @@ tsp_kernel_cfg default_cfg(const tsp_desc &d) {
     return c;
 }
 
-// Whether a step runs the fill pass before its stages. A custom boundary
-// functor or stage kernel always does. The tensor system's stage kernel reads
-// ghost memory (edges and corners for its mixed stencils): it needs the pass
-// for periodic images; without a periodic axis every ghost cell lies outside a
+// Whether a step runs the fill pass before its stages. Under a custom
+// boundary functor it always does. The tensor system's stage kernel and a
+// custom stage kernel read ghost memory: they need the pass for periodic
+// images; without a periodic axis every ghost cell lies outside a
 // non-periodic face and holds what the pass would write there (the Dirichlet
-// values written at creation, or, with NoOp, memory no stage writes). The
-// other kernels need it in ghost_mode 1.
+// values written at creation, or, with NoOp, memory no stage writes: stages
+// write valid cells only). The other kernels need it in ghost_mode 1.
 bool fills_ghosts(const tsp_solver *s) {
-    if (s->d.system == TSP_SYS_CUSTOM || s->d.bc == TSP_BC_CUSTOM) return true;
-    if (s->d.system == TSP_SYS_TENSOR_ADM) return s->d.periodic[0] || s->d.periodic[1] || s->d.periodic[2];
+    if (s->d.bc == TSP_BC_CUSTOM) return true;
+    if (s->d.system == TSP_SYS_TENSOR_ADM || s->d.system == TSP_SYS_CUSTOM)
+        return s->d.periodic[0] || s->d.periodic[1] || s->d.periodic[2];
     return s->cfg.ghost_mode == 1;
 }
 
@@ -726,7 +727,8 @@ tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, 
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW], &ARG = s->role[TSP_ROLE_ARG],
         &ARGN = s->role[TSP_ROLE_ARG_NEXT];
     for (int64_t n = 0; n < nsteps; ++n) {
-        if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f; // pre_step FillPatch
+        if (fills_ghosts(s))
+            if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f; // pre_step FillPatch
         for (int stage = 0; stage < 4; ++stage) {
             // integrators/SingleLevelRK4Integrator.hpp:115-125
             const double a_next = dt * kAFrac[(stage + 1) % 4];
@@ -734,7 +736,8 @@ tsp_status tsp_rk4_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn, 
             const double b = dt * kBFrac[stage];
             if (stage > 0) {
                 std::swap(ARG, ARGN);
-                if (tsp_status f = fill_patch(s, ARG, time_arg); f != TSP_OK) return f;
+                if (fills_ghosts(s))
+                    if (tsp_status f = fill_patch(s, ARG, time_arg); f != TSP_OK) return f;
             }
             const tsp_stage_view v =
                 stage_view(s, stage, stage == 0 ? OLD : ARG, OLD, NEW, stage < 3 ? ARGN : -1, time_arg, b, a_next);
@@ -752,7 +755,8 @@ tsp_status tsp_euler_steps_custom(tsp_solver *s, int64_t nsteps, tsp_stage_fn fn
     if (nsteps < 0) return fail(TSP_INVALID_ARG, "nsteps must be >= 0");
     int &OLD = s->role[TSP_ROLE_OLD], &NEW = s->role[TSP_ROLE_NEW];
     for (int64_t n = 0; n < nsteps; ++n) {
-        if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f;
+        if (fills_ghosts(s))
+            if (tsp_status f = fill_patch(s, OLD, s->time); f != TSP_OK) return f;
         const tsp_stage_view v = stage_view(s, tsp::kEuler, OLD, OLD, NEW, -1, s->time, s->d.dt, 0.0);
         tsp_status st = call_stage(s, fn, user, v);
         if (st != TSP_OK) return st;
