@@ -379,8 +379,8 @@ template <int R, int KY, int KG> struct TensorKO {
     static_assert(R >= 1 && KY >= R, "the y halo rows are loaded by the first and last R rows of threads");
     static_assert(6 % KG == 0, "components are taken in equal groups");
 
-    // the KO term along AX from three of the stencil's values: the centre c,
-    // and u(+s), u(-s) by a functor of s (same operation order as TensorTile::ko)
+    // the KO term along AX from the centre value c and the pair sums
+    // u(+s) + u(-s), given by a functor of s (same operation order as TensorTile::ko)
     template <int AX, class F> static __device__ __forceinline__ double ko(double c, F &&at, const StageArgs &a) {
         constexpr double c0 = static_cast<double>(cx::ko_coef(R, 0));
         constexpr double norm = cx::ko_norm(R);
