@@ -9,6 +9,8 @@
 //              to show the boundary functor matters
 //   noalloc.bin, noalloc_host.bin (library_rk4): two more steps from the final
 //              state through tsp_rk4_steps_noalloc, and the same on the host
+//   refusals.txt (library_rk4): the status codes of tsp_graph_rk4_steps and of
+//              tsp_set_kernel with ghost_mode 0 on that solver
 //   meta.txt   "<integrator> <steps> <time as %.17g> <system id> <bc id>"
 //
 //   dropin_bc <outdir> [scenario ...]
@@ -170,6 +172,15 @@ static void scenario_library(const fs::path &dir, bool euler) {
         host.set_state(s);
         for (int q = 0; q < 2; ++q) host.rk4_step();
         write_bin(dir / "noalloc_host.bin", host.get_state());
+        // what the C ABI refuses under a custom boundary functor: graph replay
+        // (it would freeze the fill times) and the virtual-ghost mode
+        tsp_solver *h = integ.device_solver().handle();
+        const int graph_status = tsp_graph_rk4_steps(h, 2);
+        tsp_kernel_cfg cfg{};
+        device::check(tsp_get_kernel(h, &cfg), "tsp_get_kernel");
+        cfg.ghost_mode = 0;
+        const int mode_status = tsp_set_kernel(h, &cfg);
+        std::ofstream(dir / "refusals.txt") << graph_status << " " << mode_status << "\n";
     }
     write_bin(dir / "host.bin", host_steps(rhs, bc, geom, dt, recs, u0, steps, euler));
     write_bin(dir / "noop.bin", host_steps(rhs, bc, geom, dt, recs, u0, steps, euler, false));

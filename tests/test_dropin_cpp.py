@@ -549,7 +549,7 @@ def test_custom_boundary_program(bc_runs, scenario):
     generic kernel on a domain whose first cell is not (0, 0, 0) with an absorbing layer. The
     result matches the same functors stepped on the host (tests/cpp/host_rk4.hpp) bitwise, and
     differs from the NoOp boundary's. Two further steps through tsp_rk4_steps_noalloc under the
-    same boundary callback match the host too."""
+    same boundary callback match the host too; graph replay and ghost_mode 0 are refused."""
     d = bc_runs / scenario
     kind, steps, time, system, bc = (d / "meta.txt").read_text().split()
     assert int(bc) == 2, "the custom boundary path did not run"
@@ -566,3 +566,5 @@ def test_custom_boundary_program(bc_runs, scenario):
         na_host = np.fromfile(d / "noalloc_host.bin", dtype=np.float64)
         assert not np.array_equal(na, final)
         assert np.array_equal(na, na_host), f"noalloc: max abs diff {np.max(np.abs(na - na_host))}"
+        graph_status, mode_status = map(int, (d / "refusals.txt").read_text().split())
+        assert graph_status == _lib.UNSUPPORTED and mode_status == _lib.INVALID_ARG
